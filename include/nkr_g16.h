@@ -1,0 +1,159 @@
+/*
+ * nkr_g16.h -- C ABI of the MI355X (gfx950) Groth16 prover backend for BN254.
+ *
+ * Every entry point returns 0 on success and a non-zero status on failure;
+ * nkr_last_error() then returns a message (thread-local, valid until the next
+ * call on the same thread).  Nothing aborts across this boundary.
+ *
+ * Data layouts are the reference's in-memory layouts (no conversion):
+ *   Fr / Fq element   32 B: 4 little-endian u64 words of x*2^256 mod p
+ *                     (libff Fp_model mont_repr, fp.hpp:42)
+ *   G1 point          96 B: Jacobian X, Y, Z over Fq (libff alt_bn128_G1)
+ *   G2 point         192 B: Jacobian X, Y, Z over Fq2 = (c0, c1)
+ *   CSR matrix        u32 row_ptr[rows + 1], u32 col_idx[nnz], Fr values[nnz]
+ *                     (protocols/groth16/common/include/r1cs/spmat.hpp:37-45)
+ *   RawProof         576 B: ar, bs1, zk, qz (G1) then bs2 (G2)
+ *                     (protocols/groth16/common/include/groth16/types.hpp:47-53)
+ *   Proof            384 B: a (G1), b (G2), c (G1)  (types.hpp:55-59)
+ * Output points are normalised: Z = 1, or (0, 1, 0) for the point at infinity,
+ * exactly what libff's to_affine_coordinates() produces.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   nkr_g16_ctx_create   HipBackend::prepare
+ *                        (protocols/groth16/backends/hip/src/backend.hip:270-296;
+ *                         virtual Backend::prepare, common/include/groth16/backend.hpp:21)
+ *   nkr_g16_prove        HipBackend::prove (backend.hip:298-335;
+ *                         virtual Backend::prove, backend.hpp:23-27)
+ *   nkr_g16_blind        blind_proof (protocols/groth16/common/src/protocol.cpp:85-94),
+ *                        with r and s supplied by the caller
+ *   nkr_fr_ntt           NTTContext::{ntt,intt,coset_ntt,coset_intt}
+ *                        (operators/hip/include/ntt/ntt.hip:106-186) with libfqfft
+ *                        basic_radix2_domain semantics (basic_radix2_domain.tcc:41-74)
+ *   nkr_spmv             spMVMContext::spmvm (backends/hip/include/spmvm/spmvm.hip:53-71)
+ *   nkr_g1_msm_*,        MSMContext::load_bases / MSMContext::msm
+ *   nkr_g2_msm_*         (operators/hip/include/msm/msm.hip:61-158); result semantics of
+ *                        libff multi_exp<BDLO12> as called by the CPU backend
+ *                        (protocols/groth16/backends/cpu/src/backend.cpp:16-23)
+ *   nkr_g16_quotient     the quotient stage of Backend::prove (backend.hip:313-321;
+ *                        CPU backend.cpp:171-179)
+ *
+ * Handles are not thread-safe: calls on one handle must be serialised.
+ * Device pointers appear only in the *_dev entry points, which exist so that
+ * benchmarks can keep inputs resident in HBM.
+ */
+#ifndef NKR_G16_H
+#define NKR_G16_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *nkr_last_error(void);
+/* Library version string. */
+const char *nkr_version(void);
+/* Number of visible GPUs. */
+int nkr_device_count(int *count);
+
+/* ---------------------------------------------------------------- Groth16 */
+typedef struct {
+    uint64_t rows;      /* constraints, a power of two >= 2 */
+    uint64_t cols;      /* variables + 1 */
+    uint64_t public_inputs;
+    const uint32_t *a_row_ptr, *a_col_idx;
+    const void *a_values;
+    uint64_t a_nnz;
+    const uint32_t *b_row_ptr, *b_col_idx;
+    const void *b_values;
+    uint64_t b_nnz;
+    const uint32_t *c_row_ptr, *c_col_idx;
+    const void *c_values;
+    uint64_t c_nnz;
+} nkr_g16_circuit;
+
+typedef struct {
+    const void *a_query;    /* cols G1 */
+    const void *b_g1_query; /* cols G1 */
+    const void *b_g2_query; /* cols G2 */
+    const void *l_query;    /* cols G1 */
+    const void *h_query;    /* rows G1 */
+    const void *alpha_g1, *beta_g1, *beta_g2, *delta_g1, *delta_g2;
+} nkr_g16_pk;
+
+typedef struct nkr_g16_ctx nkr_g16_ctx;
+
+/* Uploads the circuit, normalises and window-expands the proving key in HBM
+ * and builds the NTT tables (the reference's prepare).  Host pointers. */
+int nkr_g16_ctx_create(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int device, nkr_g16_ctx **out);
+int nkr_g16_ctx_destroy(nkr_g16_ctx *ctx);
+/* Device bytes held by the context. */
+int nkr_g16_ctx_device_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes);
+
+/* Backend::prove: witness is cols Fr (z[0] = 1), raw_out receives 576 B. */
+int nkr_g16_prove(nkr_g16_ctx *ctx, const void *witness, void *raw_out);
+/* As nkr_g16_prove, also returning the intermediates (each rows Fr, natural
+ * order) when the pointers are non-NULL: A*z, B*z, C*z and H. */
+int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h);
+
+/* blind_proof with caller-supplied r, s (Fr, Montgomery).  Host arithmetic. */
+int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
+
+/* Quotient stage alone: H from A*z, B*z, C*z (rows Fr each, host pointers). */
+int nkr_g16_quotient(const void *az, const void *bz, const void *cz, uint64_t rows, void *h, int device);
+
+/* ------------------------------------------------------ standalone operators */
+/* kind: 0 FFT, 1 iFFT, 2 cosetFFT (g = 5), 3 icosetFFT.  n = 2^log_n elements. */
+int nkr_fr_ntt(void *data, uint32_t log_n, int kind, int device);               /* host memory */
+int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream);    /* device memory */
+
+/* y = M * z for one CSR matrix (host memory). */
+int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,
+             uint64_t nnz, const void *z, void *y, int device);
+
+/* MSM with prepared bases (the reference's MSMContext: load_bases, then msm).
+ * bases: n reference-layout points (any Z; infinity allowed).  c = 0 picks
+ * the window width. */
+typedef struct nkr_msm nkr_msm;
+int nkr_g1_msm_create(const void *bases, uint64_t n, uint32_t c, int device, nkr_msm **out);
+int nkr_g2_msm_create(const void *bases, uint64_t n, uint32_t c, int device, nkr_msm **out);
+/* scalars: n Fr (host); out: one point (96 B for G1, 192 B for G2). */
+int nkr_msm_run(nkr_msm *msm, const void *scalars, void *out);
+/* scalars_dev: n Fr in device memory; stream may be NULL. */
+int nkr_msm_run_dev(nkr_msm *msm, const void *scalars_dev, void *out, void *stream);
+int nkr_msm_window_bits(const nkr_msm *msm, uint32_t *c);
+int nkr_msm_destroy(nkr_msm *msm);
+
+/* One-shot convenience: variable_base_msm(bases, scalars) semantics
+ * (operators/include/nkr/operators/variable_base_msm.hpp:17-49). */
+int nkr_g1_msm(const void *bases, const void *scalars, uint64_t n, void *out, int device);
+int nkr_g2_msm(const void *bases, const void *scalars, uint64_t n, void *out, int device);
+
+/* Fixed-base batch scalar multiplication out[i] = scalars[i] * base, the
+ * building block of key generation (FixedBaseMsm,
+ * operators/include/nkr/operators/fixed_base_msm.hpp:14-39).  base: one
+ * host point (96 B / 192 B); outputs normalised points.  The _dev forms take
+ * device scalars and a device output array; stream may be NULL. */
+int nkr_g1_fixed_base_mul(const void *base, const void *scalars, uint64_t n, void *out, int device);
+int nkr_g2_fixed_base_mul(const void *base, const void *scalars, uint64_t n, void *out, int device);
+int nkr_g1_fixed_base_mul_dev(const void *base, const void *scalars_dev, uint64_t n, void *out_dev, void *stream);
+int nkr_g2_fixed_base_mul_dev(const void *base, const void *scalars_dev, uint64_t n, void *out_dev, void *stream);
+
+/* Batch field products on the device, for arithmetic parity checks:
+ * op 0 Fr mul, 1 Fr add, 2 Fr sub, 3 Fq mul, 4 Fq add, 5 Fq sub, 6 Fq2 mul,
+ * 7 Fq2 sqr (b ignored), 8 Fr inv (b ignored).  Host memory. */
+int nkr_field_op(int op, const void *a, const void *b, void *out, uint64_t n, int device);
+
+/* Device memory helpers so FFI callers can stage data without another runtime. */
+int nkr_dev_alloc(uint64_t bytes, void **ptr);
+int nkr_dev_free(void *ptr);
+int nkr_memcpy_htod(void *dst_dev, const void *src, uint64_t bytes);
+int nkr_memcpy_dtoh(void *dst, const void *src_dev, uint64_t bytes);
+int nkr_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,459 @@
+// extern "C" boundary (include/nkr_g16.h).  Every entry point converts
+// exceptions into a non-zero status plus a thread-local message; nothing
+// aborts across the ABI (the reference's HIP_CHECK aborts,
+// operators/hip/include/utils.hip:22-31).
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/nkr_g16.h"
+#include "msm.h"
+#include "ntt.h"
+#include "prover.h"
+#include "setup.h"
+#include "spmv.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr int kMaxDevices = 16;
+std::mutex g_ntt_mutex;
+nkr::NttDomain *g_ntt_cache[kMaxDevices][29] = {};
+nkr::DeviceBuffer<nkr::fr_t> *g_ntt_scratch[kMaxDevices] = {};
+
+template <class Fn>
+int guarded(Fn &&fn)
+{
+    try {
+        fn();
+        g_last_error.clear();
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "unknown error";
+    }
+    return 1;
+}
+
+void set_device(int device)
+{
+    int count = 0;
+    NKR_HIP(hipGetDeviceCount(&count));
+    NKR_REQUIRE(device >= 0 && device < count, "invalid device index " + std::to_string(device));
+    NKR_HIP(hipSetDevice(device));
+}
+
+using namespace nkr;
+
+__global__ void k_field_op(int op, const uint8_t *a, const uint8_t *b, uint8_t *out, uint64_t n)
+{
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (op <= 2 || op == 8) {
+        const fr_t x = reinterpret_cast<const fr_t *>(a)[i];
+        const fr_t y = (op == 8) ? x : reinterpret_cast<const fr_t *>(b)[i];
+        fr_t r;
+        if (op == 0) r = fp_mul(x, y);
+        else if (op == 1) r = fp_add(x, y);
+        else if (op == 2) r = fp_sub(x, y);
+        else r = fp_inv(x);
+        reinterpret_cast<fr_t *>(out)[i] = r;
+    } else if (op <= 5) {
+        const fq_t x = reinterpret_cast<const fq_t *>(a)[i];
+        const fq_t y = reinterpret_cast<const fq_t *>(b)[i];
+        fq_t r;
+        if (op == 3) r = fp_mul(x, y);
+        else if (op == 4) r = fp_add(x, y);
+        else r = fp_sub(x, y);
+        reinterpret_cast<fq_t *>(out)[i] = r;
+    } else {
+        const fq2_t x = reinterpret_cast<const fq2_t *>(a)[i];
+        if (op == 6) {
+            const fq2_t y = reinterpret_cast<const fq2_t *>(b)[i];
+            reinterpret_cast<fq2_t *>(out)[i] = fq2_mul(x, y);
+        } else {
+            reinterpret_cast<fq2_t *>(out)[i] = fq2_sqr(x);
+        }
+    }
+}
+
+}  // namespace
+
+struct nkr_g16_ctx {
+    std::unique_ptr<nkr::Groth16Prover> prover;
+};
+
+struct nkr_msm {
+    int device = 0;
+    bool g2 = false;
+    uint64_t n = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<nkr::MsmContext<nkr::fq_t>> g1;
+    std::unique_ptr<nkr::MsmContext<nkr::fq2_t>> g2ctx;
+    nkr::DeviceBuffer<nkr::fr_t> scalars;
+    ~nkr_msm()
+    {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char *nkr_last_error(void) { return g_last_error.c_str(); }
+
+const char *nkr_version(void) { return "nkr-g16-mi355x 0.1 (gfx950)"; }
+
+int nkr_device_count(int *count)
+{
+    return guarded([&] {
+        NKR_REQUIRE(count, "null output");
+        NKR_HIP(hipGetDeviceCount(count));
+    });
+}
+
+int nkr_g16_ctx_create(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int device, nkr_g16_ctx **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(circuit && pk && out, "null argument");
+        *out = nullptr;
+        set_device(device);
+        auto ctx = std::make_unique<nkr_g16_ctx>();
+        ctx->prover = std::make_unique<Groth16Prover>(*circuit, *pk, device);
+        *out = ctx.release();
+    });
+}
+
+int nkr_g16_ctx_destroy(nkr_g16_ctx *ctx)
+{
+    return guarded([&] { delete ctx; });
+}
+
+int nkr_g16_ctx_device_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx && bytes, "null argument");
+        *bytes = ctx->prover->device_bytes();
+    });
+}
+
+int nkr_g16_prove(nkr_g16_ctx *ctx, const void *witness, void *raw_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx, "null context");
+        ctx->prover->prove(witness, raw_out, nullptr, nullptr, nullptr, nullptr);
+    });
+}
+
+int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx, "null context");
+        ctx->prover->prove(witness, raw_out, az, bz, cz, h);
+    });
+}
+
+int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(pk && raw && r && s && proof_out, "null argument");
+        fr_t rr, ss;
+        std::memcpy(&rr, r, sizeof(rr));
+        std::memcpy(&ss, s, sizeof(ss));
+        groth16_blind(*pk, raw, rr, ss, proof_out);
+    });
+}
+
+int nkr_g16_quotient(const void *az, const void *bz, const void *cz, uint64_t rows, void *h, int device)
+{
+    return guarded([&] {
+        NKR_REQUIRE(az && bz && cz && h, "null argument");
+        NKR_REQUIRE(rows >= 2 && is_pow2(rows) && rows <= (1ull << 28), "rows must be a power of two in 2..2^28");
+        set_device(device);
+        NttDomain dom(ceil_log2(rows), nullptr);
+        const size_t bytes = rows * sizeof(fr_t);
+        DeviceBuffer<fr_t> a(rows), b(rows), c(rows);
+        NKR_HIP(hipMemcpy(a.get(), az, bytes, hipMemcpyHostToDevice));
+        NKR_HIP(hipMemcpy(b.get(), bz, bytes, hipMemcpyHostToDevice));
+        NKR_HIP(hipMemcpy(c.get(), cz, bytes, hipMemcpyHostToDevice));
+        fr_t *polys[3] = {a.get(), b.get(), c.get()};
+        for (fr_t *p : polys) dom.dif(p, true, nullptr, nullptr, nullptr, nullptr);
+        for (fr_t *p : polys) dom.dit(p, false, dom.coset_rev_scaled(), nullptr, nullptr);
+        dom.quotient(a.get(), b.get(), c.get(), a.get(), nullptr);
+        dom.dif(a.get(), true, nullptr, dom.icoset_rev_scaled(), nullptr, nullptr);
+        dom.bitrev(a.get(), b.get(), nullptr);
+        NKR_HIP(hipMemcpy(h, b.get(), bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int nkr_fr_ntt(void *data, uint32_t log_n, int kind, int device)
+{
+    return guarded([&] {
+        NKR_REQUIRE(data, "null data");
+        NKR_REQUIRE(log_n >= 1 && log_n <= 28, "log_n must be 1..28");
+        set_device(device);
+        NttDomain dom(log_n, nullptr);
+        const size_t n = size_t(1) << log_n;
+        DeviceBuffer<fr_t> d(n), scratch(n);
+        NKR_HIP(hipMemcpy(d.get(), data, n * sizeof(fr_t), hipMemcpyHostToDevice));
+        dom.standalone(d.get(), scratch.get(), kind, nullptr);
+        NKR_HIP(hipMemcpy(data, d.get(), n * sizeof(fr_t), hipMemcpyDeviceToHost));
+    });
+}
+
+int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(data_dev, "null data");
+        NKR_REQUIRE(log_n >= 1 && log_n <= 28, "log_n must be 1..28");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // Domains are cached per (device, size) for repeated device-side calls.
+        // They live until the process ends: freeing device memory from static
+        // destructors would race the HIP runtime's own teardown.
+        int dev = 0;
+        NKR_HIP(hipGetDevice(&dev));
+        NKR_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+        std::lock_guard<std::mutex> lock(g_ntt_mutex);
+        NttDomain *&dom = g_ntt_cache[dev][log_n];
+        if (!dom) dom = new NttDomain(log_n, s);
+        DeviceBuffer<fr_t> *&scratch = g_ntt_scratch[dev];
+        if (!scratch) scratch = new DeviceBuffer<fr_t>();
+        scratch->ensure(size_t(1) << log_n);
+        dom->standalone(static_cast<fr_t *>(data_dev), scratch->get(), kind, s);
+    });
+}
+
+int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,
+             uint64_t nnz, const void *z, void *y, int device)
+{
+    return guarded([&] {
+        NKR_REQUIRE(row_ptr && z && y && (nnz == 0 || (col_idx && values)), "null argument");
+        NKR_REQUIRE(rows < (1ull << 32) && nnz < (1ull << 32) && cols < (1ull << 32), "matrix too large");
+        NKR_REQUIRE(row_ptr[0] == 0 && row_ptr[rows] == nnz, "invalid CSR offsets");
+        for (uint64_t r = 0; r < rows; ++r) NKR_REQUIRE(row_ptr[r + 1] >= row_ptr[r], "non-monotonic CSR offsets");
+        for (uint64_t j = 0; j < nnz; ++j) NKR_REQUIRE(col_idx[j] < cols, "out-of-range column");
+        set_device(device);
+        DeviceBuffer<uint32_t> rp(rows + 1), ci(std::max<uint64_t>(nnz, 1));
+        DeviceBuffer<fr_t> v(std::max<uint64_t>(nnz, 1)), dz(std::max<uint64_t>(cols, 1)), dy(std::max<uint64_t>(rows, 1));
+        NKR_HIP(hipMemcpy(rp.get(), row_ptr, (rows + 1) * 4, hipMemcpyHostToDevice));
+        if (nnz) {
+            NKR_HIP(hipMemcpy(ci.get(), col_idx, nnz * 4, hipMemcpyHostToDevice));
+            NKR_HIP(hipMemcpy(v.get(), values, nnz * sizeof(fr_t), hipMemcpyHostToDevice));
+        }
+        if (cols) NKR_HIP(hipMemcpy(dz.get(), z, cols * sizeof(fr_t), hipMemcpyHostToDevice));
+        CsrView m{rp.get(), ci.get(), v.get(), (uint32_t)rows, (uint32_t)nnz};
+        spmv(m, dz.get(), dy.get(), nullptr);
+        if (rows) NKR_HIP(hipMemcpy(y, dy.get(), rows * sizeof(fr_t), hipMemcpyDeviceToHost));
+    });
+}
+
+static int msm_create(const void *bases, uint64_t n, uint32_t c, int device, bool g2, nkr_msm **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(bases && out, "null argument");
+        NKR_REQUIRE(n >= 1 && n < (1ull << 31), "MSM size must be 1 .. 2^31-1");
+        *out = nullptr;
+        set_device(device);
+        auto m = std::make_unique<nkr_msm>();
+        m->device = device;
+        m->g2 = g2;
+        m->n = n;
+        NKR_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+        if (!g2) {
+            DeviceBuffer<g1_jac> tmp(n);
+            NKR_HIP(hipMemcpyAsync(tmp.get(), bases, n * sizeof(g1_jac), hipMemcpyHostToDevice, m->stream));
+            m->g1 = std::make_unique<MsmContext<fq_t>>(tmp.get(), (uint32_t)n, c, m->stream);
+            NKR_HIP(hipStreamSynchronize(m->stream));
+        } else {
+            DeviceBuffer<g2_jac> tmp(n);
+            NKR_HIP(hipMemcpyAsync(tmp.get(), bases, n * sizeof(g2_jac), hipMemcpyHostToDevice, m->stream));
+            m->g2ctx = std::make_unique<MsmContext<fq2_t>>(tmp.get(), (uint32_t)n, c, m->stream);
+            NKR_HIP(hipStreamSynchronize(m->stream));
+        }
+        m->scalars.allocate(n);
+        *out = m.release();
+    });
+}
+
+int nkr_g1_msm_create(const void *bases, uint64_t n, uint32_t c, int device, nkr_msm **out)
+{
+    return msm_create(bases, n, c, device, false, out);
+}
+
+int nkr_g2_msm_create(const void *bases, uint64_t n, uint32_t c, int device, nkr_msm **out)
+{
+    return msm_create(bases, n, c, device, true, out);
+}
+
+static void msm_run_dev(nkr_msm *m, const fr_t *scalars_dev, void *out, hipStream_t s)
+{
+    if (!m->g2) {
+        const g1_jac r = m->g1->run(scalars_dev, s);
+        std::memcpy(out, &r, sizeof(r));
+    } else {
+        const g2_jac r = m->g2ctx->run(scalars_dev, s);
+        std::memcpy(out, &r, sizeof(r));
+    }
+}
+
+int nkr_msm_run(nkr_msm *m, const void *scalars, void *out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(m && scalars && out, "null argument");
+        NKR_HIP(hipSetDevice(m->device));
+        NKR_HIP(hipMemcpyAsync(m->scalars.get(), scalars, m->n * sizeof(fr_t), hipMemcpyHostToDevice, m->stream));
+        msm_run_dev(m, m->scalars.get(), out, m->stream);
+    });
+}
+
+int nkr_msm_run_dev(nkr_msm *m, const void *scalars_dev, void *out, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(m && scalars_dev && out, "null argument");
+        NKR_HIP(hipSetDevice(m->device));
+        msm_run_dev(m, static_cast<const fr_t *>(scalars_dev), out,
+                    stream ? static_cast<hipStream_t>(stream) : m->stream);
+    });
+}
+
+int nkr_msm_window_bits(const nkr_msm *m, uint32_t *c)
+{
+    return guarded([&] {
+        NKR_REQUIRE(m && c, "null argument");
+        *c = m->g2 ? m->g2ctx->plan().c : m->g1->plan().c;
+    });
+}
+
+int nkr_msm_destroy(nkr_msm *m)
+{
+    return guarded([&] { delete m; });
+}
+
+int nkr_g1_msm(const void *bases, const void *scalars, uint64_t n, void *out, int device)
+{
+    nkr_msm *m = nullptr;
+    int rc = nkr_g1_msm_create(bases, n, 0, device, &m);
+    if (rc == 0) rc = nkr_msm_run(m, scalars, out);
+    if (m) {
+        const std::string keep = g_last_error;
+        nkr_msm_destroy(m);
+        g_last_error = keep;
+    }
+    return rc;
+}
+
+int nkr_g2_msm(const void *bases, const void *scalars, uint64_t n, void *out, int device)
+{
+    nkr_msm *m = nullptr;
+    int rc = nkr_g2_msm_create(bases, n, 0, device, &m);
+    if (rc == 0) rc = nkr_msm_run(m, scalars, out);
+    if (m) {
+        const std::string keep = g_last_error;
+        nkr_msm_destroy(m);
+        g_last_error = keep;
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+template <class F>
+static int fixed_base_mul(const void *base, const void *scalars, uint64_t n, void *out, int device)
+{
+    return guarded([&] {
+        NKR_REQUIRE(base && scalars && out, "null argument");
+        if (n == 0) return;
+        set_device(device);
+        Jacobian<F> b;
+        std::memcpy(&b, base, sizeof(b));
+        FixedBaseMul<F> fbm(b, nullptr);
+        DeviceBuffer<fr_t> ds(n);
+        DeviceBuffer<Jacobian<F>> dout(n);
+        NKR_HIP(hipMemcpy(ds.get(), scalars, n * sizeof(fr_t), hipMemcpyHostToDevice));
+        fbm.run(ds.get(), n, dout.get(), nullptr);
+        NKR_HIP(hipMemcpy(out, dout.get(), n * sizeof(Jacobian<F>), hipMemcpyDeviceToHost));
+    });
+}
+
+template <class F>
+static int fixed_base_mul_dev(const void *base, const void *scalars_dev, uint64_t n, void *out_dev, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(base && scalars_dev && out_dev, "null argument");
+        if (n == 0) return;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        Jacobian<F> b;
+        std::memcpy(&b, base, sizeof(b));
+        FixedBaseMul<F> fbm(b, s);
+        fbm.run(static_cast<const fr_t *>(scalars_dev), n, static_cast<Jacobian<F> *>(out_dev), s);
+        NKR_HIP(hipStreamSynchronize(s));
+    });
+}
+
+extern "C" {
+
+int nkr_g1_fixed_base_mul(const void *base, const void *scalars, uint64_t n, void *out, int device)
+{
+    return fixed_base_mul<fq_t>(base, scalars, n, out, device);
+}
+int nkr_g2_fixed_base_mul(const void *base, const void *scalars, uint64_t n, void *out, int device)
+{
+    return fixed_base_mul<fq2_t>(base, scalars, n, out, device);
+}
+int nkr_g1_fixed_base_mul_dev(const void *base, const void *scalars_dev, uint64_t n, void *out_dev, void *stream)
+{
+    return fixed_base_mul_dev<fq_t>(base, scalars_dev, n, out_dev, stream);
+}
+int nkr_g2_fixed_base_mul_dev(const void *base, const void *scalars_dev, uint64_t n, void *out_dev, void *stream)
+{
+    return fixed_base_mul_dev<fq2_t>(base, scalars_dev, n, out_dev, stream);
+}
+
+int nkr_field_op(int op, const void *a, const void *b, void *out, uint64_t n, int device)
+{
+    return guarded([&] {
+        NKR_REQUIRE(op >= 0 && op <= 8, "unknown field op");
+        NKR_REQUIRE(a && out && (b || op == 7 || op == 8), "null argument");
+        if (n == 0) return;
+        set_device(device);
+        const size_t esz = (op == 6 || op == 7) ? 64 : 32;
+        DeviceBuffer<uint8_t> da(n * esz), db(n * esz), dout(n * esz);
+        NKR_HIP(hipMemcpy(da.get(), a, n * esz, hipMemcpyHostToDevice));
+        if (b && op != 7 && op != 8) NKR_HIP(hipMemcpy(db.get(), b, n * esz, hipMemcpyHostToDevice));
+        k_field_op<<<div_up(n, 256), 256>>>(op, da.get(), db.get(), dout.get(), n);
+        NKR_LAUNCH_CHECK();
+        NKR_HIP(hipMemcpy(out, dout.get(), n * esz, hipMemcpyDeviceToHost));
+    });
+}
+
+int nkr_dev_alloc(uint64_t bytes, void **ptr)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ptr, "null argument");
+        NKR_HIP(hipMalloc(ptr, bytes));
+    });
+}
+
+int nkr_dev_free(void *ptr)
+{
+    return guarded([&] { NKR_HIP(hipFree(ptr)); });
+}
+
+int nkr_memcpy_htod(void *dst, const void *src, uint64_t bytes)
+{
+    return guarded([&] { NKR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+
+int nkr_memcpy_dtoh(void *dst, const void *src, uint64_t bytes)
+{
+    return guarded([&] { NKR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+
+int nkr_device_synchronize(void)
+{
+    return guarded([&] { NKR_HIP(hipDeviceSynchronize()); });
+}
+
+}  // extern "C"

@@ -1,0 +1,275 @@
+// BN254 G1 (over Fq) and G2 (over Fq2) point arithmetic for gfx950.
+//
+// Buckets and partial sums use extended Jacobian "XYZZ" coordinates
+// (x = X/ZZ, y = Y/ZZZ, ZZ^3 = ZZZ^2); the point at infinity is ZZ = 0.
+// Bases are affine (x, y) with an explicit infinity flag kept beside them.
+//
+// Every addition here is complete for the inputs a Pippenger bucket can see:
+// P + P falls through to a doubling and P + (-P) to infinity.  (The
+// reference's HIP bucket kernel uses an addition that returns infinity for
+// P + P, operators/hip/include/curves/xyzz_t.hip:436-474; duplicate proving
+// key bases make that reachable.)
+//
+// Formulas: madd-2008-s, add-2008-s, dbl-2008-s-1 and mdbl-2008-s-1 from the
+// Explicit-Formulas Database, specialised to a = 0.
+#pragma once
+
+#include "field.h"
+
+namespace nkr {
+
+template <class F>
+struct Affine {
+    F x, y;
+};
+
+// The word that carries the infinity flag of an affine base: bit 31 of the
+// top limb of y (field elements are < 2^254, so the bit is otherwise clear).
+NKR_HD uint32_t &inf_word(fq_t &y) { return y.l[7]; }
+NKR_HD uint32_t inf_word(const fq_t &y) { return y.l[7]; }
+NKR_HD uint32_t &inf_word(fq2_t &y) { return y.c1.l[7]; }
+NKR_HD uint32_t inf_word(const fq2_t &y) { return y.c1.l[7]; }
+
+template <class F>
+NKR_HD Affine<F> affine_infinity()
+{
+    Affine<F> a;
+    a.x = FieldOps<F>::zero();
+    a.y = FieldOps<F>::zero();
+    inf_word(a.y) = 0x80000000u;
+    return a;
+}
+
+template <class F>
+NKR_HD bool affine_is_inf(const Affine<F> &a)
+{
+    return (inf_word(a.y) >> 31) != 0;
+}
+
+template <class F>
+struct Xyzz {
+    F X, Y, ZZ, ZZZ;
+
+    NKR_HD static Xyzz infinity()
+    {
+        Xyzz r;
+        r.X = FieldOps<F>::zero();
+        r.Y = FieldOps<F>::zero();
+        r.ZZ = FieldOps<F>::zero();
+        r.ZZZ = FieldOps<F>::zero();
+        return r;
+    }
+    NKR_HD bool is_inf() const { return ZZ.is_zero(); }
+    NKR_HD static Xyzz from_affine(const Affine<F> &p)
+    {
+        Xyzz r;
+        r.X = p.x;
+        r.Y = p.y;
+        r.ZZ = FieldOps<F>::one();
+        r.ZZZ = FieldOps<F>::one();
+        return r;
+    }
+};
+
+// 2*(x, y) for an affine point (mdbl-2008-s-1).
+template <class F>
+NKR_HD Xyzz<F> xyzz_dbl_affine(const F &x, const F &y)
+{
+    using O = FieldOps<F>;
+    const F U = O::dbl(y);
+    const F V = O::sqr(U);
+    const F W = O::mul(U, V);
+    const F S = O::mul(x, V);
+    const F x2 = O::sqr(x);
+    const F M = O::add(O::dbl(x2), x2);
+    Xyzz<F> r;
+    r.X = O::sub(O::sqr(M), O::dbl(S));
+    r.Y = O::sub(O::mul(M, O::sub(S, r.X)), O::mul(W, y));
+    r.ZZ = V;
+    r.ZZZ = W;
+    return r;
+}
+
+// 2*P (dbl-2008-s-1); infinity stays infinity.
+template <class F>
+NKR_HD Xyzz<F> xyzz_dbl(const Xyzz<F> &p)
+{
+    using O = FieldOps<F>;
+    const F U = O::dbl(p.Y);
+    const F V = O::sqr(U);
+    const F W = O::mul(U, V);
+    const F S = O::mul(p.X, V);
+    const F x2 = O::sqr(p.X);
+    const F M = O::add(O::dbl(x2), x2);
+    Xyzz<F> r;
+    r.X = O::sub(O::sqr(M), O::dbl(S));
+    r.Y = O::sub(O::mul(M, O::sub(S, r.X)), O::mul(W, p.Y));
+    r.ZZ = O::mul(V, p.ZZ);
+    r.ZZZ = O::mul(W, p.ZZZ);
+    return r;  // p = infinity gives ZZ = 0 again
+}
+
+// acc + (x2, y2), complete (madd-2008-s with the equal / opposite cases).
+template <class F>
+NKR_HD Xyzz<F> xyzz_add_affine(const Xyzz<F> &acc, const F &x2, const F &y2)
+{
+    using O = FieldOps<F>;
+    if (acc.is_inf()) {
+        Xyzz<F> r;
+        r.X = x2;
+        r.Y = y2;
+        r.ZZ = O::one();
+        r.ZZZ = O::one();
+        return r;
+    }
+    const F U2 = O::mul(x2, acc.ZZ);
+    const F S2 = O::mul(y2, acc.ZZZ);
+    const F P = O::sub(U2, acc.X);
+    const F R = O::sub(S2, acc.Y);
+    if (P.is_zero()) {
+        if (R.is_zero()) return xyzz_dbl_affine(x2, y2);
+        return Xyzz<F>::infinity();
+    }
+    const F PP = O::sqr(P);
+    const F PPP = O::mul(P, PP);
+    const F Qv = O::mul(acc.X, PP);
+    Xyzz<F> r;
+    r.X = O::sub(O::sub(O::sqr(R), PPP), O::dbl(Qv));
+    r.Y = O::sub(O::mul(R, O::sub(Qv, r.X)), O::mul(acc.Y, PPP));
+    r.ZZ = O::mul(acc.ZZ, PP);
+    r.ZZZ = O::mul(acc.ZZZ, PPP);
+    return r;
+}
+
+// p + q, complete (add-2008-s with the equal / opposite / infinity cases).
+template <class F>
+NKR_HD Xyzz<F> xyzz_add(const Xyzz<F> &p, const Xyzz<F> &q)
+{
+    using O = FieldOps<F>;
+    if (p.is_inf()) return q;
+    if (q.is_inf()) return p;
+    const F U1 = O::mul(p.X, q.ZZ);
+    const F U2 = O::mul(q.X, p.ZZ);
+    const F S1 = O::mul(p.Y, q.ZZZ);
+    const F S2 = O::mul(q.Y, p.ZZZ);
+    const F P = O::sub(U2, U1);
+    const F R = O::sub(S2, S1);
+    if (P.is_zero()) {
+        if (R.is_zero()) return xyzz_dbl(p);
+        return Xyzz<F>::infinity();
+    }
+    const F PP = O::sqr(P);
+    const F PPP = O::mul(P, PP);
+    const F Qv = O::mul(U1, PP);
+    Xyzz<F> r;
+    r.X = O::sub(O::sub(O::sqr(R), PPP), O::dbl(Qv));
+    r.Y = O::sub(O::mul(R, O::sub(Qv, r.X)), O::mul(S1, PPP));
+    r.ZZ = O::mul(O::mul(p.ZZ, q.ZZ), PP);
+    r.ZZZ = O::mul(O::mul(p.ZZZ, q.ZZZ), PPP);
+    return r;
+}
+
+template <class F>
+NKR_HD Xyzz<F> xyzz_neg(const Xyzz<F> &p)
+{
+    Xyzz<F> r = p;
+    r.Y = FieldOps<F>::neg(p.Y);
+    return r;
+}
+
+// Jacobian (X, Y, Z) with x = X/Z^2, y = Y/Z^3: the reference's host layout
+// (libff alt_bn128_G1 / alt_bn128_G2: 3 coordinates, infinity is Z = 0).
+template <class F>
+struct Jacobian {
+    F X, Y, Z;
+};
+
+// Affine image in the reference's to_affine_coordinates form: (x, y, 1), or
+// (0, 1, 0) for infinity (alt_bn128_g1.cpp:64-80).
+template <class F>
+NKR_HD Jacobian<F> xyzz_to_jacobian_affine(const Xyzz<F> &p)
+{
+    using O = FieldOps<F>;
+    Jacobian<F> r;
+    if (p.is_inf()) {
+        r.X = O::zero();
+        r.Y = O::one();
+        r.Z = O::zero();
+        return r;
+    }
+    // x = X/ZZ, y = Y/ZZZ with one inversion: t = 1/(ZZ*ZZZ).
+    const F t = O::inv(O::mul(p.ZZ, p.ZZZ));
+    const F zz_inv = O::mul(t, p.ZZZ);
+    const F zzz_inv = O::mul(t, p.ZZ);
+    r.X = O::mul(p.X, zz_inv);
+    r.Y = O::mul(p.Y, zzz_inv);
+    r.Z = O::one();
+    return r;
+}
+
+template <class F>
+NKR_HD Affine<F> xyzz_to_affine(const Xyzz<F> &p)
+{
+    using O = FieldOps<F>;
+    if (p.is_inf()) return affine_infinity<F>();
+    const F t = O::inv(O::mul(p.ZZ, p.ZZZ));
+    Affine<F> r;
+    r.x = O::mul(p.X, O::mul(t, p.ZZZ));
+    r.y = O::mul(p.Y, O::mul(t, p.ZZ));
+    return r;
+}
+
+// Jacobian -> affine; returns false for infinity.
+template <class F>
+NKR_HD bool jacobian_to_affine(const Jacobian<F> &p, Affine<F> &out)
+{
+    using O = FieldOps<F>;
+    if (p.Z.is_zero()) {
+        out.x = O::zero();
+        out.y = O::zero();
+        return false;
+    }
+    const F zi = O::inv(p.Z);
+    const F zi2 = O::sqr(zi);
+    out.x = O::mul(p.X, zi2);
+    out.y = O::mul(p.Y, O::mul(zi2, zi));
+    return true;
+}
+
+template <class F>
+NKR_HD Xyzz<F> jacobian_to_xyzz(const Jacobian<F> &p)
+{
+    using O = FieldOps<F>;
+    if (p.Z.is_zero()) return Xyzz<F>::infinity();
+    Xyzz<F> r;
+    r.X = p.X;
+    r.Y = p.Y;
+    r.ZZ = O::sqr(p.Z);
+    r.ZZZ = O::mul(r.ZZ, p.Z);
+    return r;
+}
+
+// k*P for a 256-bit little-endian integer k, left-to-right double and add.
+template <class F>
+NKR_HD Xyzz<F> xyzz_mul_u256(const Xyzz<F> &p, const uint32_t k[8])
+{
+    Xyzz<F> acc = Xyzz<F>::infinity();
+    for (int w = 7; w >= 0; --w)
+        for (int b = 31; b >= 0; --b) {
+            acc = xyzz_dbl(acc);
+            if ((k[w] >> b) & 1u) acc = xyzz_add(acc, p);
+        }
+    return acc;
+}
+
+using g1_affine = Affine<fq_t>;
+using g2_affine = Affine<fq2_t>;
+using g1_xyzz = Xyzz<fq_t>;
+using g2_xyzz = Xyzz<fq2_t>;
+using g1_jac = Jacobian<fq_t>;
+using g2_jac = Jacobian<fq2_t>;
+
+static_assert(sizeof(g1_jac) == 96, "G1 Jacobian must match the reference's 96-byte layout");
+static_assert(sizeof(g2_jac) == 192, "G2 Jacobian must match the reference's 192-byte layout");
+
+}  // namespace nkr

@@ -1,0 +1,78 @@
+// Radix-2 number-theoretic transforms over Fr for one power-of-two domain.
+//
+// A transform of N = 2^n points runs as 1-3 passes.  Each pass performs
+// 2^np-point sub-transforms entirely in LDS (one workgroup holds up to 2048
+// elements = 64 KiB) and multiplies by the inter-pass twiddles omega^(lo*k*2^S)
+// (the four-step factorisation), which come from two small tables
+// (omega^e = hi[e >> m] * lo[e & (2^m - 1)]) that stay cache resident.
+//
+// dif(): natural order in, bit-reversed order out (decimation in frequency).
+// dit(): bit-reversed order in, natural order out; it is the transpose of
+//        dif() and uses the same twiddles.
+// The Groth16 quotient chain composes the two so that no bit-reversal pass is
+// ever needed; the standalone transforms add one gather pass to return
+// natural order like libfqfft's basic_radix2_domain.
+#pragma once
+
+#include <vector>
+
+#include "field.h"
+#include "runtime.h"
+
+namespace nkr {
+
+class NttDomain {
+public:
+    NttDomain(unsigned log_n, hipStream_t stream);
+
+    unsigned log_n() const { return log_n_; }
+    size_t size() const { return size_t(1) << log_n_; }
+
+    // In-place transforms on device data.  `pre` / `post` are optional
+    // per-position multipliers (indexed by storage position), `post_const` an
+    // optional constant factor applied last.
+    void dif(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, const fr_t *post_const,
+             hipStream_t stream) const;
+    void dit(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, hipStream_t stream) const;
+
+    // out[i] = in[bitrev(i)] (out of place).
+    void bitrev(const fr_t *in, fr_t *out, hipStream_t stream) const;
+
+    // libfqfft semantics, natural order in and out (kind: 0 FFT, 1 iFFT,
+    // 2 cosetFFT, 3 icosetFFT), using `scratch` (n elements) for the final
+    // reordering.
+    void standalone(fr_t *data, fr_t *scratch, int kind, hipStream_t stream) const;
+
+    // h[i] = (a[i]*b[i] - c[i]) / (g^N - 1) on the coset (divide_by_Z_on_coset).
+    void quotient(const fr_t *a, const fr_t *b, const fr_t *c, fr_t *h, hipStream_t stream) const;
+
+    // Tables the prove chain fuses into its passes.
+    const fr_t *coset_rev_scaled() const { return coset_rev_.get(); }       // g^rev(j) / N
+    const fr_t *icoset_rev_scaled() const { return icoset_rev_.get(); }     // g^-rev(j) / N
+
+    fr_t omega() const { return omega_[0]; }
+
+private:
+    struct Pass {
+        unsigned S, np, log_L, log_cols;
+    };
+    void run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const fr_t *post, const fr_t *post_const,
+             hipStream_t stream) const;
+
+    unsigned log_n_;
+    unsigned m_lo_;        // bits of the low twiddle table index
+    unsigned log_roots_;   // sub-transform roots are of order 2^log_roots_
+    std::vector<Pass> passes_;
+    fr_t omega_[2];        // omega, omega^-1
+    fr_t n_inv_;
+    fr_t zinv_;            // (g^N - 1)^-1
+    DeviceBuffer<fr_t> tw_lo_[2], tw_hi_[2], roots_[2];
+    DeviceBuffer<fr_t> coset_nat_, coset_rev_, icoset_rev_, consts_;
+};
+
+// Fr constants shared by host code.
+fr_t fr_from_u64(uint64_t x);
+fr_t fr_root_of_unity(unsigned log_n);   // primitive 2^log_n-th root (libff field_utils rule)
+fr_t fr_pow_u64(const fr_t &x, uint64_t e);
+
+}  // namespace nkr

@@ -1,0 +1,342 @@
+// Fr NTT kernels for gfx950: LDS-resident sub-transforms with four-step
+// inter-pass twiddles.  Algorithmic reference: libfqfft basic_radix2_domain
+// (third_party/lightzk/depends/libsnark/depends/libfqfft/libfqfft/
+// evaluation_domain/domains/basic_radix2_domain.tcc:41-112 and
+// basic_radix2_domain_aux.tcc:45-76,172-180): the same omega, coset g = 5 and
+// 1/N scaling, so outputs equal the reference's element for element.
+#include "ntt.h"
+
+namespace nkr {
+
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+struct PassArgs {
+    fr_t *data;
+    const fr_t *tw_lo;
+    const fr_t *tw_hi;
+    const fr_t *roots;
+    const fr_t *pre;
+    const fr_t *post;
+    fr_t post_const;
+    uint32_t has_post_const;
+    uint32_t log_n, S, np, log_L, log_cols, log_roots, m_lo;
+    uint32_t twiddle;
+};
+
+__device__ __forceinline__ uint32_t brev_bits(uint32_t x, uint32_t bits)
+{
+    return bits == 0 ? 0u : (__brev(x) >> (32 - bits));
+}
+
+// omega^e from the two-level table.
+__device__ __forceinline__ fr_t tw_pow(const PassArgs &a, uint32_t e)
+{
+    const uint32_t lo = e & ((1u << a.m_lo) - 1u);
+    const uint32_t hi = e >> a.m_lo;
+    return fp_mul(a.tw_hi[hi], a.tw_lo[lo]);
+}
+
+template <bool DIT>
+__global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs a)
+{
+    extern __shared__ fr_t smem[];
+    const uint32_t G = 1u << a.np;
+    const uint32_t cols = 1u << a.log_cols;
+    const uint32_t total = G << a.log_cols;
+    const uint32_t L = 1u << a.log_L;
+    const bool col_fast = L >= cols;  // consecutive groups are consecutive addresses
+    const uint64_t g0 = (uint64_t)blockIdx.x << a.log_cols;
+
+    for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        uint32_t j, c;
+        if (col_fast) {
+            c = idx & (cols - 1);
+            j = idx >> a.log_cols;
+        } else {
+            j = idx & (G - 1);
+            c = idx >> a.np;
+        }
+        const uint64_t g = g0 + c;
+        const uint64_t hi = g >> a.log_L;
+        const uint32_t lo = (uint32_t)(g & (L - 1));
+        const uint64_t pos = (hi << (a.log_n - a.S)) + ((uint64_t)j << a.log_L) + lo;
+        fr_t v = a.data[pos];
+        if (a.pre) v = fp_mul(v, a.pre[pos]);
+        if (DIT && a.twiddle) {
+            const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
+            if (e) v = fp_mul(v, tw_pow(a, e));
+        }
+        smem[j * cols + c] = v;
+    }
+    __syncthreads();
+
+    const uint32_t half = total >> 1;
+    const uint32_t root_shift = a.log_roots - a.np;
+    for (uint32_t step = 0; step < a.np; ++step) {
+        const uint32_t t = DIT ? (a.np - 1 - step) : step;
+        const uint32_t log_h = a.np - 1 - t;
+        const uint32_t h = 1u << log_h;
+        for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
+            const uint32_t c = b & (cols - 1);
+            const uint32_t q = b >> a.log_cols;
+            const uint32_t jl = ((q >> log_h) << (log_h + 1)) | (q & (h - 1));
+            const uint32_t jh = jl + h;
+            const uint32_t e = (q & (h - 1)) << t;
+            const fr_t x = smem[jl * cols + c];
+            const fr_t y = smem[jh * cols + c];
+            if (!DIT) {
+                fr_t d = fp_sub(x, y);
+                if (e) d = fp_mul(d, a.roots[e << root_shift]);
+                smem[jl * cols + c] = fp_add(x, y);
+                smem[jh * cols + c] = d;
+            } else {
+                const fr_t wy = e ? fp_mul(y, a.roots[e << root_shift]) : y;
+                smem[jl * cols + c] = fp_add(x, wy);
+                smem[jh * cols + c] = fp_sub(x, wy);
+            }
+        }
+        __syncthreads();
+    }
+
+    for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+        uint32_t j, c;
+        if (col_fast) {
+            c = idx & (cols - 1);
+            j = idx >> a.log_cols;
+        } else {
+            j = idx & (G - 1);
+            c = idx >> a.np;
+        }
+        const uint64_t g = g0 + c;
+        const uint64_t hi = g >> a.log_L;
+        const uint32_t lo = (uint32_t)(g & (L - 1));
+        const uint64_t pos = (hi << (a.log_n - a.S)) + ((uint64_t)j << a.log_L) + lo;
+        fr_t v = smem[j * cols + c];
+        if (!DIT && a.twiddle) {
+            const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
+            if (e) v = fp_mul(v, tw_pow(a, e));
+        }
+        if (a.post) v = fp_mul(v, a.post[pos]);
+        if (a.has_post_const) v = fp_mul(v, a.post_const);
+        a.data[pos] = v;
+    }
+}
+
+// out[e] = base^e for e < count.
+__global__ void k_pow_table(fr_t *out, fr_t base, uint32_t count)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    fr_t acc = fr_t::one();
+    fr_t sq = base;
+    for (uint32_t x = e; x; x >>= 1) {
+        if (x & 1) acc = fp_mul(acc, sq);
+        sq = fp_sqr(sq);
+    }
+    out[e] = acc;
+}
+
+// out[j] = base^bitrev(j) * scale for j < 2^log_n.
+__global__ void k_pow_rev_table(fr_t *out, fr_t base, fr_t scale, uint32_t log_n)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (1u << log_n)) return;
+    fr_t acc = scale;
+    fr_t sq = base;
+    for (uint32_t x = brev_bits(j, log_n); x; x >>= 1) {
+        if (x & 1) acc = fp_mul(acc, sq);
+        sq = fp_sqr(sq);
+    }
+    out[j] = acc;
+}
+
+__global__ void k_bitrev(const fr_t *__restrict__ in, fr_t *__restrict__ out, uint32_t log_n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (1u << log_n)) return;
+    out[i] = in[brev_bits(i, log_n)];
+}
+
+__global__ void k_quotient(const fr_t *__restrict__ a, const fr_t *__restrict__ b, const fr_t *__restrict__ c,
+                           fr_t *__restrict__ h, fr_t zinv, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    h[i] = fp_mul(fp_sub(fp_mul(a[i], b[i]), c[i]), zinv);
+}
+
+// 2^28-th root of unity of Fr (libff alt_bn128_init.cpp:56, as an integer).
+const uint32_t kRootOfUnity28[8] = {0x725b19f0u, 0x9bd61b6eu, 0x41112ed4u, 0x402d111eu,
+                                    0x8ef62abcu, 0x00e0a7ebu, 0xa58a7e85u, 0x2a3c09f0u};
+constexpr unsigned kTwoAdicity = 28;
+
+}  // namespace
+
+fr_t fr_from_u64(uint64_t x)
+{
+    fr_t t = fr_t::zero();
+    t.l[0] = (uint32_t)x;
+    t.l[1] = (uint32_t)(x >> 32);
+    return fp_to_mont(t);
+}
+
+fr_t fr_pow_u64(const fr_t &x, uint64_t e)
+{
+    fr_t acc = fr_t::one(), sq = x;
+    for (; e; e >>= 1) {
+        if (e & 1) acc = fp_mul(acc, sq);
+        sq = fp_sqr(sq);
+    }
+    return acc;
+}
+
+fr_t fr_root_of_unity(unsigned log_n)
+{
+    NKR_REQUIRE(log_n <= kTwoAdicity, "domain exceeds the 2-adicity of Fr (2^28)");
+    fr_t w;
+    for (int i = 0; i < 8; ++i) w.l[i] = kRootOfUnity28[i];
+    w = fp_to_mont(w);
+    for (unsigned i = kTwoAdicity; i > log_n; --i) w = fp_sqr(w);
+    return w;
+}
+
+NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
+{
+    NKR_REQUIRE(log_n >= 1 && log_n <= kTwoAdicity, "NTT size must be 2^1 .. 2^28");
+    const size_t n = size_t(1) << log_n;
+    omega_[0] = fr_root_of_unity(log_n);
+    omega_[1] = fp_inv(omega_[0]);
+    n_inv_ = fp_inv(fr_from_u64(n));
+    const fr_t g = fr_from_u64(5);  // multiplicative generator (alt_bn128_init.cpp:55)
+    const fr_t g_inv = fp_inv(g);
+    zinv_ = fp_inv(fp_sub(fr_pow_u64(g, n), fr_t::one()));
+
+    // pass plan: sub-transforms of at most 11 bits (2048 elements)
+    std::vector<unsigned> split;
+    if (log_n <= 11) {
+        split = {log_n};
+    } else if (log_n <= 22) {
+        split = {(log_n + 1) / 2, log_n / 2};
+    } else {
+        const unsigned a = (log_n + 2) / 3, b = (log_n + 1) / 3, c = log_n / 3;
+        split = {a, b, c};
+    }
+    unsigned S = 0;
+    for (unsigned np : split) {
+        Pass p;
+        p.S = S;
+        p.np = np;
+        p.log_L = log_n - S - np;
+        const unsigned groups_log = log_n - np;
+        unsigned lc = std::min(groups_log, 11u - np);
+        if (p.log_L > 0 && p.log_L < lc) lc = p.log_L;
+        p.log_cols = lc;
+        passes_.push_back(p);
+        S += np;
+    }
+
+    m_lo_ = (log_n + 1) / 2;
+    log_roots_ = std::min(log_n, 12u);
+    const uint32_t lo_count = 1u << m_lo_;
+    const uint32_t hi_count = 1u << (log_n - m_lo_);
+    const uint32_t root_count = std::max(1u, 1u << (log_roots_ - 1));
+    for (int d = 0; d < 2; ++d) {
+        tw_lo_[d].allocate(lo_count);
+        tw_hi_[d].allocate(hi_count);
+        roots_[d].allocate(root_count);
+        k_pow_table<<<div_up(lo_count, 256), 256, 0, stream>>>(tw_lo_[d].get(), omega_[d], lo_count);
+        NKR_LAUNCH_CHECK();
+        k_pow_table<<<div_up(hi_count, 256), 256, 0, stream>>>(tw_hi_[d].get(), fr_pow_u64(omega_[d], lo_count),
+                                                              hi_count);
+        NKR_LAUNCH_CHECK();
+        const fr_t root = fr_pow_u64(omega_[d], uint64_t(1) << (log_n - log_roots_));
+        k_pow_table<<<div_up(root_count, 256), 256, 0, stream>>>(roots_[d].get(), root, root_count);
+        NKR_LAUNCH_CHECK();
+    }
+    coset_nat_.allocate(n);
+    coset_rev_.allocate(n);
+    icoset_rev_.allocate(n);
+    k_pow_table<<<div_up(n, 256), 256, 0, stream>>>(coset_nat_.get(), g, (uint32_t)n);
+    NKR_LAUNCH_CHECK();
+    k_pow_rev_table<<<div_up(n, 256), 256, 0, stream>>>(coset_rev_.get(), g, n_inv_, log_n);
+    NKR_LAUNCH_CHECK();
+    k_pow_rev_table<<<div_up(n, 256), 256, 0, stream>>>(icoset_rev_.get(), g_inv, n_inv_, log_n);
+    NKR_LAUNCH_CHECK();
+}
+
+void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const fr_t *post,
+                    const fr_t *post_const, hipStream_t stream) const
+{
+    const int d = inverse ? 1 : 0;
+    const int P = (int)passes_.size();
+    for (int k = 0; k < P; ++k) {
+        const int pi = dit ? (P - 1 - k) : k;
+        const Pass &p = passes_[pi];
+        PassArgs a{};
+        a.data = data;
+        a.tw_lo = tw_lo_[d].get();
+        a.tw_hi = tw_hi_[d].get();
+        a.roots = roots_[d].get();
+        a.pre = (k == 0) ? pre : nullptr;
+        a.post = (k == P - 1) ? post : nullptr;
+        a.has_post_const = (k == P - 1 && post_const) ? 1u : 0u;
+        if (a.has_post_const) a.post_const = *post_const;
+        a.log_n = log_n_;
+        a.S = p.S;
+        a.np = p.np;
+        a.log_L = p.log_L;
+        a.log_cols = p.log_cols;
+        a.log_roots = log_roots_;
+        a.m_lo = m_lo_;
+        a.twiddle = (pi != P - 1) ? 1u : 0u;
+        const uint32_t blocks = (uint32_t)((size_t(1) << (log_n_ - p.np)) >> p.log_cols);
+        const size_t lds = (size_t(sizeof(fr_t)) << p.np) << p.log_cols;
+        if (dit)
+            k_ntt_pass<true><<<blocks, kThreads, lds, stream>>>(a);
+        else
+            k_ntt_pass<false><<<blocks, kThreads, lds, stream>>>(a);
+        NKR_LAUNCH_CHECK();
+    }
+}
+
+void NttDomain::dif(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, const fr_t *post_const,
+                    hipStream_t stream) const
+{
+    run(data, false, inverse, pre, post, post_const, stream);
+}
+
+void NttDomain::dit(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, hipStream_t stream) const
+{
+    run(data, true, inverse, pre, post, nullptr, stream);
+}
+
+void NttDomain::bitrev(const fr_t *in, fr_t *out, hipStream_t stream) const
+{
+    const size_t n = size();
+    k_bitrev<<<div_up(n, 256), 256, 0, stream>>>(in, out, log_n_);
+    NKR_LAUNCH_CHECK();
+}
+
+void NttDomain::standalone(fr_t *data, fr_t *scratch, int kind, hipStream_t stream) const
+{
+    switch (kind) {
+    case 0: dif(data, false, nullptr, nullptr, nullptr, stream); break;
+    case 1: dif(data, true, nullptr, nullptr, &n_inv_, stream); break;
+    case 2: dif(data, false, coset_nat_.get(), nullptr, nullptr, stream); break;
+    case 3: dif(data, true, nullptr, icoset_rev_.get(), nullptr, stream); break;
+    default: throw Error("unknown transform kind (0 FFT, 1 iFFT, 2 cosetFFT, 3 icosetFFT)");
+    }
+    bitrev(data, scratch, stream);
+    NKR_HIP(hipMemcpyAsync(data, scratch, size() * sizeof(fr_t), hipMemcpyDeviceToDevice, stream));
+}
+
+void NttDomain::quotient(const fr_t *a, const fr_t *b, const fr_t *c, fr_t *h, hipStream_t stream) const
+{
+    const size_t n = size();
+    k_quotient<<<div_up(n, 256), 256, 0, stream>>>(a, b, c, h, zinv_, (uint32_t)n);
+    NKR_LAUNCH_CHECK();
+}
+
+}  // namespace nkr

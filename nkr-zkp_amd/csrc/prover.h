@@ -1,0 +1,60 @@
+// Groth16 prove orchestration on one GPU: the MI355X counterpart of the
+// reference's HipBackend (protocols/groth16/backends/hip/src/backend.hip:
+// 270-335) and of CpuBackend::prove (backends/cpu/src/backend.cpp:156-201).
+//
+// prepare (constructor): CSR matrices and proving key to HBM, key points
+// normalised and window-expanded once (MsmBases), NTT tables built.
+// prove: witness H2D -> A*z, B*z, C*z -> iNTT x3 -> coset NTT x3 -> (ab - c)/Z
+// -> coset iNTT (H left in bit-reversed order; the H-query bases are stored
+// in the same order) -> one signed-digit sort of the witness shared by the
+// ar / bs1 / zk / bs2 MSMs -> one for H -> bit planes D2H -> host finish.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../../include/nkr_g16.h"
+#include "msm.h"
+#include "ntt.h"
+#include "spmv.h"
+
+namespace nkr {
+
+class Groth16Prover {
+public:
+    Groth16Prover(const nkr_g16_circuit &circuit, const nkr_g16_pk &pk, int device);
+    ~Groth16Prover();
+    Groth16Prover(const Groth16Prover &) = delete;
+    Groth16Prover &operator=(const Groth16Prover &) = delete;
+
+    // raw_out: 576 B.  az/bz/cz/h: optional host outputs (rows Fr, natural order).
+    void prove(const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h);
+
+    uint64_t device_bytes() const;
+    int device() const { return device_; }
+    hipStream_t stream() const { return stream_; }
+
+private:
+    int device_;
+    hipStream_t stream_ = nullptr;
+    uint32_t rows_, cols_, log_n_;
+    DeviceBuffer<uint32_t> row_ptr_[3], col_idx_[3];
+    DeviceBuffer<fr_t> values_[3];
+    CsrView views_[3];
+    std::unique_ptr<NttDomain> domain_;
+    MsmPlan wplan_, hplan_;
+    MsmBases<fq_t> a_, b1_, l_, h_;
+    MsmBases<fq2_t> b2_;
+    MsmSchedule wsched_, hsched_;
+    MsmRunner<fq_t> run1_;
+    MsmRunner<fq2_t> run2_;
+    DeviceBuffer<fr_t> z_, pa_, pb_, pc_;
+    DeviceBuffer<Xyzz<fq_t>> planes1_;
+    DeviceBuffer<Xyzz<fq2_t>> planes2_;
+    void *pinned_witness_ = nullptr;
+};
+
+// blind_proof (protocols/groth16/common/src/protocol.cpp:85-94) with given r, s.
+void groth16_blind(const nkr_g16_pk &pk, const void *raw, const fr_t &r, const fr_t &s, void *proof_out);
+
+}  // namespace nkr

@@ -1,0 +1,244 @@
+#include "prover.h"
+
+#include <cstring>
+
+namespace nkr {
+
+namespace {
+
+std::vector<uint32_t> bitrev_permutation(unsigned log_n)
+{
+    const size_t n = size_t(1) << log_n;
+    std::vector<uint32_t> p(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t r = 0, x = (uint32_t)i;
+        for (unsigned b = 0; b < log_n; ++b) {
+            r = (r << 1) | (x & 1u);
+            x >>= 1;
+        }
+        p[i] = r;
+    }
+    return p;
+}
+
+void check_csr(uint64_t rows, uint64_t cols, const uint32_t *rp, const uint32_t *ci, const void *vals, uint64_t nnz,
+               const char *name)
+{
+    const std::string m(name);
+    NKR_REQUIRE(rp && (nnz == 0 || (ci && vals)), m + ": null CSR array");
+    NKR_REQUIRE(rp[0] == 0 && rp[rows] == nnz, m + " has invalid CSR offsets");
+    for (uint64_t r = 0; r < rows; ++r) NKR_REQUIRE(rp[r + 1] >= rp[r], m + " has non-monotonic CSR offsets");
+    for (uint64_t j = 0; j < nnz; ++j) NKR_REQUIRE(ci[j] < cols, m + " contains an out-of-range column");
+}
+
+template <class T>
+void upload(DeviceBuffer<T> &buf, const void *src, size_t count, hipStream_t stream)
+{
+    buf.allocate(std::max<size_t>(count, 1));
+    if (count) NKR_HIP(hipMemcpyAsync(buf.get(), src, count * sizeof(T), hipMemcpyHostToDevice, stream));
+}
+
+}  // namespace
+
+Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int device) : device_(device)
+{
+    // Same preconditions as validate_circuit / validate_proving_key
+    // (protocols/groth16/common/src/protocol.cpp:9-67).
+    NKR_REQUIRE(c.rows >= 2 && is_pow2(c.rows), "constraint count must be a power of two (>= 2)");
+    NKR_REQUIRE(c.rows <= (1ull << 28), "constraint count exceeds the 2-adicity of Fr");
+    NKR_REQUIRE(c.cols >= 1 && c.cols < (1ull << 31), "invalid variable count");
+    NKR_REQUIRE(c.public_inputs + 1 <= c.cols, "public input count exceeds the variable count");
+    check_csr(c.rows, c.cols, c.a_row_ptr, c.a_col_idx, c.a_values, c.a_nnz, "A");
+    check_csr(c.rows, c.cols, c.b_row_ptr, c.b_col_idx, c.b_values, c.b_nnz, "B");
+    check_csr(c.rows, c.cols, c.c_row_ptr, c.c_col_idx, c.c_values, c.c_nnz, "C");
+    NKR_REQUIRE(pk.a_query && pk.b_g1_query && pk.b_g2_query && pk.l_query && pk.h_query,
+                "proving key query vectors missing");
+
+    rows_ = (uint32_t)c.rows;
+    cols_ = (uint32_t)c.cols;
+    log_n_ = ceil_log2(rows_);
+    NKR_HIP(hipSetDevice(device_));
+    NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+    const uint32_t *rps[3] = {c.a_row_ptr, c.b_row_ptr, c.c_row_ptr};
+    const uint32_t *cis[3] = {c.a_col_idx, c.b_col_idx, c.c_col_idx};
+    const void *vs[3] = {c.a_values, c.b_values, c.c_values};
+    const uint64_t nnz[3] = {c.a_nnz, c.b_nnz, c.c_nnz};
+    for (int m = 0; m < 3; ++m) {
+        NKR_REQUIRE(nnz[m] < (1ull << 32), "matrix has too many non-zeros");
+        upload(row_ptr_[m], rps[m], rows_ + 1, stream_);
+        upload(col_idx_[m], cis[m], nnz[m], stream_);
+        upload(values_[m], vs[m], nnz[m], stream_);
+        views_[m] = CsrView{row_ptr_[m].get(), col_idx_[m].get(), values_[m].get(), rows_, (uint32_t)nnz[m]};
+    }
+
+    domain_ = std::make_unique<NttDomain>(log_n_, stream_);
+    wplan_ = MsmPlan::choose(cols_);
+    hplan_ = MsmPlan::choose(rows_);
+
+    {
+        DeviceBuffer<g1_jac> tmp1(std::max(cols_, rows_));
+        const void *g1_vecs[3] = {pk.a_query, pk.b_g1_query, pk.l_query};
+        MsmBases<fq_t> *g1_sets[3] = {&a_, &b1_, &l_};
+        for (int k = 0; k < 3; ++k) {
+            NKR_HIP(hipMemcpyAsync(tmp1.get(), g1_vecs[k], (size_t)cols_ * sizeof(g1_jac), hipMemcpyHostToDevice,
+                                   stream_));
+            g1_sets[k]->prepare(tmp1.get(), nullptr, wplan_, stream_);
+            NKR_HIP(hipStreamSynchronize(stream_));
+        }
+        // H-query in bit-reversed order, matching the order H leaves the chain in.
+        const std::vector<uint32_t> perm = bitrev_permutation(log_n_);
+        DeviceBuffer<uint32_t> dperm;
+        upload(dperm, perm.data(), perm.size(), stream_);
+        NKR_HIP(hipMemcpyAsync(tmp1.get(), pk.h_query, (size_t)rows_ * sizeof(g1_jac), hipMemcpyHostToDevice,
+                               stream_));
+        h_.prepare(tmp1.get(), dperm.get(), hplan_, stream_);
+        NKR_HIP(hipStreamSynchronize(stream_));
+    }
+    {
+        DeviceBuffer<g2_jac> tmp2(cols_);
+        NKR_HIP(hipMemcpyAsync(tmp2.get(), pk.b_g2_query, (size_t)cols_ * sizeof(g2_jac), hipMemcpyHostToDevice,
+                               stream_));
+        b2_.prepare(tmp2.get(), nullptr, wplan_, stream_);
+        NKR_HIP(hipStreamSynchronize(stream_));
+    }
+
+    z_.allocate(cols_);
+    pa_.allocate(rows_);
+    pb_.allocate(rows_);
+    pc_.allocate(rows_);
+    planes1_.allocate(3 * wplan_.c + hplan_.c);
+    planes2_.allocate(wplan_.c);
+    NKR_HIP(hipHostMalloc(&pinned_witness_, (size_t)cols_ * sizeof(fr_t), hipHostMallocDefault));
+    NKR_HIP(hipStreamSynchronize(stream_));
+}
+
+Groth16Prover::~Groth16Prover()
+{
+    if (pinned_witness_) (void)hipHostFree(pinned_witness_);
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+}
+
+uint64_t Groth16Prover::device_bytes() const
+{
+    uint64_t b = a_.bytes() + b1_.bytes() + l_.bytes() + h_.bytes() + b2_.bytes();
+    for (int m = 0; m < 3; ++m) b += row_ptr_[m].bytes() + col_idx_[m].bytes() + values_[m].bytes();
+    b += z_.bytes() + pa_.bytes() + pb_.bytes() + pc_.bytes();
+    return b;
+}
+
+void Groth16Prover::prove(const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h)
+{
+    NKR_REQUIRE(witness && raw_out, "null witness or output");
+    NKR_HIP(hipSetDevice(device_));
+    hipStream_t s = stream_;
+    const size_t fr_bytes = (size_t)cols_ * sizeof(fr_t);
+    std::memcpy(pinned_witness_, witness, fr_bytes);
+    NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, s));
+
+    spmv3(views_[0], views_[1], views_[2], z_.get(), pa_.get(), pb_.get(), pc_.get(), s);
+    const size_t poly_bytes = (size_t)rows_ * sizeof(fr_t);
+    if (az) NKR_HIP(hipMemcpyAsync(az, pa_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
+    if (bz) NKR_HIP(hipMemcpyAsync(bz, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
+    if (cz) NKR_HIP(hipMemcpyAsync(cz, pc_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
+
+    // quotient chain, no bit-reversal passes
+    fr_t *polys[3] = {pa_.get(), pb_.get(), pc_.get()};
+    for (fr_t *p : polys) domain_->dif(p, true, nullptr, nullptr, nullptr, s);
+    for (fr_t *p : polys) domain_->dit(p, false, domain_->coset_rev_scaled(), nullptr, s);
+    domain_->quotient(pa_.get(), pb_.get(), pc_.get(), pa_.get(), s);
+    domain_->dif(pa_.get(), true, nullptr, domain_->icoset_rev_scaled(), nullptr, s);
+    if (h) {
+        domain_->bitrev(pa_.get(), pb_.get(), s);
+        NKR_HIP(hipMemcpyAsync(h, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
+    }
+
+    // MSMs
+    wsched_.build(z_.get(), wplan_, s);
+    const uint32_t cw = wplan_.c;
+    run1_.run(a_, wsched_, planes1_.get() + 0 * cw, s);
+    run1_.run(b1_, wsched_, planes1_.get() + 1 * cw, s);
+    run1_.run(l_, wsched_, planes1_.get() + 2 * cw, s);
+    run2_.run(b2_, wsched_, planes2_.get(), s);
+    hsched_.build(pa_.get(), hplan_, s);
+    run1_.run(h_, hsched_, planes1_.get() + 3 * cw, s);
+
+    std::vector<Xyzz<fq_t>> p1(3 * cw + hplan_.c);
+    std::vector<Xyzz<fq2_t>> p2(cw);
+    NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
+    NKR_HIP(hipMemcpyAsync(p2.data(), planes2_.get(), p2.size() * sizeof(p2[0]), hipMemcpyDeviceToHost, s));
+    NKR_HIP(hipStreamSynchronize(s));
+
+    uint8_t *out = static_cast<uint8_t *>(raw_out);
+    const g1_jac ar = msm_finish(p1.data() + 0 * cw, cw);
+    const g1_jac bs1 = msm_finish(p1.data() + 1 * cw, cw);
+    const g1_jac zk = msm_finish(p1.data() + 2 * cw, cw);
+    const g1_jac qz = msm_finish(p1.data() + 3 * cw, hplan_.c);
+    const g2_jac bs2 = msm_finish(p2.data(), cw);
+    std::memcpy(out + 0, &ar, 96);
+    std::memcpy(out + 96, &bs1, 96);
+    std::memcpy(out + 192, &zk, 96);
+    std::memcpy(out + 288, &qz, 96);
+    std::memcpy(out + 384, &bs2, 192);
+}
+
+namespace {
+
+template <class F>
+Xyzz<F> jmul(const Jacobian<F> &p, const fr_t &k_mont)
+{
+    const fr_t k = fp_from_mont(k_mont);
+    return xyzz_mul_u256(jacobian_to_xyzz(p), k.l);
+}
+
+template <class F>
+Jacobian<F> load_point(const void *p)
+{
+    Jacobian<F> j;
+    std::memcpy(&j, p, sizeof(j));
+    return j;
+}
+
+}  // namespace
+
+void groth16_blind(const nkr_g16_pk &pk, const void *raw, const fr_t &r, const fr_t &s, void *proof_out)
+{
+    NKR_REQUIRE(pk.alpha_g1 && pk.beta_g1 && pk.beta_g2 && pk.delta_g1 && pk.delta_g2,
+                "proving key group elements missing");
+    const uint8_t *rw = static_cast<const uint8_t *>(raw);
+    const g1_xyzz ar = jacobian_to_xyzz(load_point<fq_t>(rw + 0));
+    const g1_xyzz bs1 = jacobian_to_xyzz(load_point<fq_t>(rw + 96));
+    const g1_xyzz zk = jacobian_to_xyzz(load_point<fq_t>(rw + 192));
+    const g1_xyzz qz = jacobian_to_xyzz(load_point<fq_t>(rw + 288));
+    const g2_xyzz bs2 = jacobian_to_xyzz(load_point<fq2_t>(rw + 384));
+    const g1_jac alpha = load_point<fq_t>(pk.alpha_g1);
+    const g1_jac beta1 = load_point<fq_t>(pk.beta_g1);
+    const g1_jac delta1 = load_point<fq_t>(pk.delta_g1);
+    const g2_jac beta2 = load_point<fq2_t>(pk.beta_g2);
+    const g2_jac delta2 = load_point<fq2_t>(pk.delta_g2);
+
+    // A = alpha + ar + r*delta1
+    const g1_xyzz a = xyzz_add(xyzz_add(jacobian_to_xyzz(alpha), ar), jmul(delta1, r));
+    // B1 = beta1 + bs1 + s*delta1 ; B = beta2 + bs2 + s*delta2
+    const g1_xyzz b1 = xyzz_add(xyzz_add(jacobian_to_xyzz(beta1), bs1), jmul(delta1, s));
+    const g2_xyzz b2 = xyzz_add(xyzz_add(jacobian_to_xyzz(beta2), bs2), jmul(delta2, s));
+    // C = qz + zk + s*A + r*B1 - (r*s)*delta1
+    const fr_t sk = fp_from_mont(s), rk = fp_from_mont(r);
+    g1_xyzz cpt = xyzz_add(qz, zk);
+    cpt = xyzz_add(cpt, xyzz_mul_u256(a, sk.l));
+    cpt = xyzz_add(cpt, xyzz_mul_u256(b1, rk.l));
+    cpt = xyzz_add(cpt, xyzz_neg(jmul(delta1, fp_mul(r, s))));
+
+    const g1_jac A = xyzz_to_jacobian_affine(a);
+    const g2_jac B = xyzz_to_jacobian_affine(b2);
+    const g1_jac C = xyzz_to_jacobian_affine(cpt);
+    uint8_t *o = static_cast<uint8_t *>(proof_out);
+    std::memcpy(o, &A, 96);
+    std::memcpy(o + 96, &B, 192);
+    std::memcpy(o + 288, &C, 96);
+}
+
+}  // namespace nkr

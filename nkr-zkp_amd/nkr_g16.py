@@ -1,0 +1,313 @@
+"""ctypes binding of libnkr_g16.so, the gfx950 Groth16 backend (C ABI in
+include/nkr_g16.h).  Used by the tests, the smoke check and the benchmark.
+
+There is no fallback: if the shared library is missing, importing the
+functions raises.  Arrays are numpy uint64 in the reference's in-memory
+layouts (Fr/Fq 4 words, G1 Jacobian 12 words, G2 Jacobian 24 words).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "lib", "libnkr_g16.so")
+HEADER = os.path.join(ROOT, "include", "nkr_g16.h")
+
+
+class NkrError(RuntimeError):
+    pass
+
+
+class Circuit(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_uint64), ("cols", ctypes.c_uint64), ("public_inputs", ctypes.c_uint64),
+                ("a_row_ptr", ctypes.c_void_p), ("a_col_idx", ctypes.c_void_p), ("a_values", ctypes.c_void_p),
+                ("a_nnz", ctypes.c_uint64),
+                ("b_row_ptr", ctypes.c_void_p), ("b_col_idx", ctypes.c_void_p), ("b_values", ctypes.c_void_p),
+                ("b_nnz", ctypes.c_uint64),
+                ("c_row_ptr", ctypes.c_void_p), ("c_col_idx", ctypes.c_void_p), ("c_values", ctypes.c_void_p),
+                ("c_nnz", ctypes.c_uint64)]
+
+
+class Pk(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
+        "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2")]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NkrError(f"{LIB_PATH} is not built (run __graft_entry__.build() or make -C nkr-zkp_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int
+        L.nkr_last_error.restype = ctypes.c_char_p
+        L.nkr_version.restype = ctypes.c_char_p
+        sigs = {
+            "nkr_device_count": [ctypes.POINTER(ctypes.c_int)],
+            "nkr_g16_ctx_create": [ctypes.POINTER(Circuit), ctypes.POINTER(Pk), i32, ctypes.POINTER(vp)],
+            "nkr_g16_ctx_destroy": [vp],
+            "nkr_g16_ctx_device_bytes": [vp, ctypes.POINTER(u64)],
+            "nkr_g16_prove": [vp, vp, vp],
+            "nkr_g16_prove_ex": [vp, vp, vp, vp, vp, vp, vp],
+            "nkr_g16_blind": [ctypes.POINTER(Pk), vp, vp, vp, vp],
+            "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
+            "nkr_fr_ntt": [vp, u32, i32, i32],
+            "nkr_fr_ntt_dev": [vp, u32, i32, vp],
+            "nkr_spmv": [u64, u64, vp, vp, vp, u64, vp, vp, i32],
+            "nkr_g1_msm_create": [vp, u64, u32, i32, ctypes.POINTER(vp)],
+            "nkr_g2_msm_create": [vp, u64, u32, i32, ctypes.POINTER(vp)],
+            "nkr_msm_run": [vp, vp, vp],
+            "nkr_msm_run_dev": [vp, vp, vp, vp],
+            "nkr_msm_window_bits": [vp, ctypes.POINTER(u32)],
+            "nkr_msm_destroy": [vp],
+            "nkr_g1_msm": [vp, vp, u64, vp, i32],
+            "nkr_g2_msm": [vp, vp, u64, vp, i32],
+            "nkr_field_op": [i32, vp, vp, vp, u64, i32],
+            "nkr_g1_fixed_base_mul": [vp, vp, u64, vp, i32],
+            "nkr_g2_fixed_base_mul": [vp, vp, u64, vp, i32],
+            "nkr_g1_fixed_base_mul_dev": [vp, vp, u64, vp, vp],
+            "nkr_g2_fixed_base_mul_dev": [vp, vp, u64, vp, vp],
+            "nkr_dev_alloc": [u64, ctypes.POINTER(vp)],
+            "nkr_dev_free": [vp],
+            "nkr_memcpy_htod": [vp, vp, u64],
+            "nkr_memcpy_dtoh": [vp, vp, u64],
+            "nkr_device_synchronize": [],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def header_functions() -> list[str]:
+    """Names of every function declared in include/nkr_g16.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(nkr_\w+)\s*\(", text, re.M)))
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise NkrError(lib().nkr_last_error().decode())
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _words(a, width: int) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64).reshape(-1, width))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().nkr_device_count(ctypes.byref(n)))
+    return n.value
+
+
+FIELD_OPS = {"fr_mul": 0, "fr_add": 1, "fr_sub": 2, "fq_mul": 3, "fq_add": 4, "fq_sub": 5,
+             "fq2_mul": 6, "fq2_sqr": 7, "fr_inv": 8}
+
+
+def field_op(name: str, a, b=None, device: int = 0) -> np.ndarray:
+    op = FIELD_OPS[name]
+    width = 8 if name.startswith("fq2") else 4
+    a = _words(a, width)
+    b = _words(b, width) if b is not None else None
+    out = np.zeros_like(a)
+    _check(lib().nkr_field_op(op, _p(a), _p(b) if b is not None else None, _p(out), a.shape[0], device))
+    return out
+
+
+FFT, IFFT, COSET_FFT, ICOSET_FFT = 0, 1, 2, 3
+
+
+def ntt(a, kind: int, device: int = 0) -> np.ndarray:
+    out = _words(a, 4).copy()
+    n = out.shape[0]
+    log_n = n.bit_length() - 1
+    if n != 1 << log_n:
+        raise ValueError("size must be a power of two")
+    _check(lib().nkr_fr_ntt(_p(out), log_n, kind, device))
+    return out
+
+
+def quotient(az, bz, cz, device: int = 0) -> np.ndarray:
+    az, bz, cz = (_words(v, 4) for v in (az, bz, cz))
+    h = np.zeros_like(az)
+    _check(lib().nkr_g16_quotient(_p(az), _p(bz), _p(cz), az.shape[0], _p(h), device))
+    return h
+
+
+def spmv(rows, cols, row_ptr, col_idx, values, z, device: int = 0) -> np.ndarray:
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+    values = _words(values, 4)
+    z = _words(z, 4)
+    y = np.zeros((rows, 4), dtype=np.uint64)
+    _check(lib().nkr_spmv(rows, cols, _p(row_ptr), _p(col_idx) if col_idx.size else None,
+                          _p(values) if values.size else None, values.shape[0], _p(z), _p(y), device))
+    return y
+
+
+class Msm:
+    """Prepared-bases MSM (MSMContext: load_bases once, run per scalar vector)."""
+
+    def __init__(self, bases, g2: bool = False, c: int = 0, device: int = 0):
+        self.g2 = g2
+        width = 24 if g2 else 12
+        b = _words(bases, width)
+        self.n = b.shape[0]
+        h = ctypes.c_void_p()
+        fn = lib().nkr_g2_msm_create if g2 else lib().nkr_g1_msm_create
+        _check(fn(_p(b), self.n, c, device, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def window_bits(self) -> int:
+        c = ctypes.c_uint32()
+        _check(lib().nkr_msm_window_bits(self.h, ctypes.byref(c)))
+        return c.value
+
+    def run(self, scalars) -> np.ndarray:
+        s = _words(scalars, 4)
+        assert s.shape[0] == self.n
+        out = np.zeros(24 if self.g2 else 12, dtype=np.uint64)
+        _check(lib().nkr_msm_run(self.h, _p(s), _p(out)))
+        return out
+
+    def run_dev(self, scalars_dev: int, stream: int | None = None) -> np.ndarray:
+        out = np.zeros(24 if self.g2 else 12, dtype=np.uint64)
+        _check(lib().nkr_msm_run_dev(self.h, scalars_dev, _p(out), stream))
+        return out
+
+    def close(self) -> None:
+        if self.h:
+            lib().nkr_msm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fixed_base_mul(base, scalars, g2: bool = False, device: int = 0) -> np.ndarray:
+    """out[i] = scalars[i] * base (normalised points)."""
+    width = 24 if g2 else 12
+    b = np.ascontiguousarray(np.asarray(base, dtype=np.uint64).reshape(width))
+    s = _words(scalars, 4)
+    out = np.zeros((s.shape[0], width), dtype=np.uint64)
+    fn = lib().nkr_g2_fixed_base_mul if g2 else lib().nkr_g1_fixed_base_mul
+    _check(fn(_p(b), _p(s), s.shape[0], _p(out), device))
+    return out
+
+
+def msm(bases, scalars, g2: bool = False, device: int = 0) -> np.ndarray:
+    width = 24 if g2 else 12
+    b = _words(bases, width)
+    s = _words(scalars, 4)
+    out = np.zeros(width, dtype=np.uint64)
+    fn = lib().nkr_g2_msm if g2 else lib().nkr_g1_msm
+    _check(fn(_p(b), _p(s), b.shape[0], _p(out), device))
+    return out
+
+
+class Prover:
+    """Groth16 backend context: prepare in the constructor, prove per witness."""
+
+    def __init__(self, circ, pk, device: int = 0):
+        self._keep = []
+
+        def arr(x, dtype, width=None):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
+            if width:
+                a = a.reshape(-1, width)
+            self._keep.append(a)
+            return a
+
+        mats = []
+        for m in (circ.a, circ.b, circ.c):
+            rp, ci, va = arr(m[0], np.uint32), arr(m[1], np.uint32), arr(m[2], np.uint64, 4)
+            mats.append((rp, ci, va))
+        self.rows, self.cols = circ.rows, circ.cols
+        c = Circuit(circ.rows, circ.cols, circ.public_inputs,
+                    *[v for (rp, ci, va) in mats for v in (_p(rp), _p(ci) if ci.size else None,
+                                                            _p(va) if va.size else None, va.shape[0])])
+        self.pk_struct = self._pk(pk)
+        h = ctypes.c_void_p()
+        _check(lib().nkr_g16_ctx_create(ctypes.byref(c), ctypes.byref(self.pk_struct), device, ctypes.byref(h)))
+        self.h = h
+
+    def _pk(self, pk) -> Pk:
+        arrs = {}
+        for k in ("a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
+                  "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2"):
+            a = np.ascontiguousarray(np.asarray(getattr(pk, k), dtype=np.uint64))
+            self._keep.append(a)
+            arrs[k] = a.ctypes.data
+        return Pk(**arrs)
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_uint64()
+        _check(lib().nkr_g16_ctx_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def prove(self, witness, intermediates: bool = False):
+        w = _words(witness, 4)
+        assert w.shape[0] == self.cols
+        raw = np.zeros(72, dtype=np.uint64)
+        if not intermediates:
+            _check(lib().nkr_g16_prove(self.h, _p(w), _p(raw)))
+            return raw
+        bufs = [np.zeros((self.rows, 4), dtype=np.uint64) for _ in range(4)]
+        _check(lib().nkr_g16_prove_ex(self.h, _p(w), _p(raw), *[_p(b) for b in bufs]))
+        return raw, tuple(bufs)
+
+    def blind(self, raw, r, s) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, dtype=np.uint64)
+        r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+        s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+        out = np.zeros(48, dtype=np.uint64)
+        _check(lib().nkr_g16_blind(ctypes.byref(self.pk_struct), _p(raw), _p(r), _p(s), _p(out)))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().nkr_g16_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def blind(pk, raw, r, s) -> np.ndarray:
+    keep = []
+    arrs = {}
+    for k in ("a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
+              "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2"):
+        a = np.ascontiguousarray(np.asarray(getattr(pk, k), dtype=np.uint64))
+        keep.append(a)
+        arrs[k] = a.ctypes.data
+    p = Pk(**arrs)
+    raw = np.ascontiguousarray(raw, dtype=np.uint64)
+    r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+    s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+    out = np.zeros(48, dtype=np.uint64)
+    _check(lib().nkr_g16_blind(ctypes.byref(p), _p(raw), _p(r), _p(s), _p(out)))
+    return out

@@ -113,8 +113,14 @@ def fr_array(values) -> np.ndarray:
     return np.array([int_to_words(to_mont_int(v % R, R)) for v in values], dtype=np.uint64).reshape(-1, 4)
 
 
+_RINV_R = pow(MONT, -1, R)
+
+
 def fr_ints(arr: np.ndarray) -> list[int]:
-    return [from_mont_int(words_to_int(r), R) for r in arr.reshape(-1, 4)]
+    a = np.ascontiguousarray(arr, dtype=np.uint64).reshape(-1, 4)
+    w = [a[:, i].tolist() for i in range(4)]
+    return [((w3 << 192) | (w2 << 128) | (w1 << 64) | w0) * _RINV_R % R
+            for w0, w1, w2, w3 in zip(*w)]
 
 
 # ------------------------------------------------------------------ wrappers

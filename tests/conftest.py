@@ -4,7 +4,7 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for sub in ("", "oracle", "tests"):
+for sub in ("", "oracle", "tests", "tests/golden", "nkr-zkp_amd"):
     p = os.path.join(ROOT, sub)
     if p not in sys.path:
         sys.path.insert(0, p)
@@ -12,4 +12,11 @@ for sub in ("", "oracle", "tests"):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP backend)")
-    config.addinivalue_line("markers", "slow: longer CPU-side check")
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    # Name the case before any GPU work so a fault report identifies it.
+    if request.node.get_closest_marker("gpu"):
+        print(f"\n[case] {request.node.nodeid}", flush=True)
+    yield

@@ -1,0 +1,196 @@
+"""Parity of the HIP operators against the CPU oracle and the golden vectors,
+called through the C ABI.  Integer field work: the bar is bit-exact equality."""
+import numpy as np
+import pytest
+
+import goldens
+import nkr_g16 as N
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+G = goldens.load("bn254_golden.json")
+
+
+def rand_fr(seed, n):
+    return O.fr_random(seed, n)
+
+
+def rand_fq(seed, n):
+    # uniform < q: libff rule against q, via the Fr sampler's stream then reduce
+    x = O.fr_random(seed, n)
+    return x  # every value < r < q is a valid Fq residue too
+
+
+# ---------------------------------------------------------------- field
+@pytest.mark.parametrize("field", ["fr", "fq"])
+def test_field_golden(field):
+    d = G[field]
+    a, b = goldens.fe(d["a"]), goldens.fe(d["b"])
+    assert np.array_equal(N.field_op(f"{field}_mul", a, b), goldens.fe(d["mul"]))
+    assert np.array_equal(N.field_op(f"{field}_add", a, b), goldens.fe(d["add"]))
+    assert np.array_equal(N.field_op(f"{field}_sub", a, b), goldens.fe(d["sub"]))
+    if field == "fr":
+        assert np.array_equal(N.field_op("fr_inv", a), goldens.fe(d["inv"]))
+
+
+def test_fq2_golden():
+    d = G["fq2"]
+    a = np.array([goldens.fe(x).reshape(-1) for x in d["a"]])
+    b = np.array([goldens.fe(x).reshape(-1) for x in d["b"]])
+    assert np.array_equal(N.field_op("fq2_mul", a, b), np.array([goldens.fe(x).reshape(-1) for x in d["mul"]]))
+    assert np.array_equal(N.field_op("fq2_sqr", a), np.array([goldens.fe(x).reshape(-1) for x in d["sqr"]]))
+
+
+def test_field_random_large():
+    n = 1 << 18
+    a, b = rand_fr(101, n), rand_fr(102, n)
+    assert np.array_equal(N.field_op("fr_mul", a, b), O.fr_mul(a, b))
+    assert np.array_equal(N.field_op("fq_mul", a, b), O.fq_mul(a, b))
+    assert np.array_equal(N.field_op("fr_sub", a, b), O.fr_sub(a, b))
+    a2, b2 = a[: n // 2].reshape(-1, 8), b[: n // 2].reshape(-1, 8)
+    assert np.array_equal(N.field_op("fq2_mul", a2, b2), O.fq2_mul(a2, b2))
+
+
+# ------------------------------------------------------------------ NTT
+@pytest.mark.parametrize("kind,key", [(N.FFT, "fft"), (N.IFFT, "ifft"), (N.COSET_FFT, "coset_fft"),
+                                      (N.ICOSET_FFT, "icoset_fft")])
+def test_ntt_golden(kind, key):
+    for case in G["fft"]:
+        got = N.ntt(goldens.fe(case["a"]), kind)
+        assert np.array_equal(got, goldens.fe(case[key])), (key, case["n"])
+
+
+@pytest.mark.parametrize("log_n", [11, 12, 13, 16, 20])
+def test_ntt_vs_oracle(log_n):
+    a = rand_fr(200 + log_n, 1 << log_n)
+    for kind in (N.FFT, N.IFFT, N.COSET_FFT, N.ICOSET_FFT):
+        assert np.array_equal(N.ntt(a, kind), O.fft(a, kind)), (log_n, kind)
+
+
+def test_ntt_round_trip_2_24():
+    n = 1 << 24
+    a = rand_fr(224, n)
+    f = N.ntt(a, N.FFT)
+    assert np.array_equal(N.ntt(f, N.IFFT), a)
+    c = N.ntt(a, N.COSET_FFT)
+    assert np.array_equal(N.ntt(c, N.ICOSET_FFT), a)
+    # FFT(delta_0) = all ones; FFT is linear
+    one = O.int_to_words(O.to_mont_int(1, O.R))
+    d = np.zeros((n, 4), dtype=np.uint64)
+    d[0] = one
+    ones = N.ntt(d, N.FFT)
+    assert (ones == np.array(one, dtype=np.uint64)).all()
+
+
+def test_quotient_golden_and_oracle():
+    for case in G["quotient"]:
+        h = N.quotient(goldens.fe(case["az"]), goldens.fe(case["bz"]), goldens.fe(case["cz"]))
+        assert np.array_equal(h, goldens.fe(case["h"])), case["n"]
+    n = 1 << 14
+    az, bz, cz = rand_fr(301, n), rand_fr(302, n), rand_fr(303, n)
+    assert np.array_equal(N.quotient(az, bz, cz), O.quotient(az, bz, cz))
+
+
+# ----------------------------------------------------------------- SpMV
+def test_spmv_golden():
+    d = G["spmv"]
+    y = N.spmv(d["rows"], d["cols"], np.array(d["row_ptr"]), np.array(d["col_idx"]),
+               goldens.fe(d["values"]), goldens.fe(d["z"]))
+    assert np.array_equal(y, goldens.fe(d["y"]))
+
+
+def test_spmv_random_large():
+    rows, cols = 1 << 16, 1 << 16
+    rng = np.random.default_rng(7)
+    per_row = rng.integers(0, 6, rows)
+    row_ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    nnz = int(row_ptr[-1])
+    col_idx = rng.integers(0, cols, nnz).astype(np.uint32)
+    vals = rand_fr(401, nnz)
+    z = rand_fr(402, cols)
+    got = N.spmv(rows, cols, row_ptr, col_idx, vals, z)
+    assert np.array_equal(got, O.spmv(rows, cols, row_ptr, col_idx, vals, z))
+
+
+# ------------------------------------------------------------------ MSM
+def test_fixed_base_mul_matches_oracle():
+    k = rand_fr(501, 300)
+    k[0] = 0
+    k[1] = O.int_to_words(O.to_mont_int(1, O.R))
+    g1, g2 = O.g1_generator(), O.g2_generator()
+    assert np.array_equal(N.fixed_base_mul(g1, k), O.g1_to_affine(O.g1_mul(g1, k)))
+    assert np.array_equal(N.fixed_base_mul(g2, k[:40], g2=True), O.g2_to_affine(O.g2_mul(g2, k[:40])))
+
+
+def test_msm_golden_fixtures():
+    for case in G["msm"]:
+        bases = goldens.points(case["bases"])
+        sc = goldens.fe(case["scalars"])
+        g2 = case["group"] == "g2"
+        got = N.msm(bases, sc, g2=g2)
+        assert np.array_equal(got, goldens.point(case["result"])), (case["group"], case["n"], case["flavour"])
+
+
+@pytest.mark.parametrize("n,flavour", [(1, "uniform"), (3, "uniform"), (4096, "uniform"), (4096, "witness"),
+                                       (5000, "zeros")])
+def test_msm_g1_vs_oracle(n, flavour):
+    g1 = O.g1_generator()
+    bases = N.fixed_base_mul(g1, rand_fr(600 + n, n))
+    s = rand_fr(700 + n, n)
+    if flavour == "witness":
+        rng = np.random.default_rng(n)
+        pick = rng.random(n) < 0.5
+        bit = rng.random(n) < 0.5
+        one = np.array(O.int_to_words(O.to_mont_int(1, O.R)), dtype=np.uint64)
+        s[pick & bit] = one
+        s[pick & ~bit] = 0
+    if flavour == "zeros":
+        s[:] = 0
+        s[17] = O.int_to_words(O.to_mont_int(5, O.R))
+    got = N.msm(bases, s)
+    want = O.g1_to_affine(O.msm_g1(bases, s))[0]
+    assert np.array_equal(got, want)
+
+
+def test_msm_duplicate_and_opposite_bases():
+    """P + P and P + (-P) inside one bucket (the reference HIP kernel's
+    incomplete addition returns infinity for the former)."""
+    g1 = O.g1_generator()
+    base = N.fixed_base_mul(g1, rand_fr(801, 1))[0]
+    neg = O.g1_to_affine(O.g1_neg(base))[0]
+    bases = np.stack([base] * 40 + [neg] * 7)
+    s = np.repeat(rand_fr(802, 1), 47, axis=0)
+    got = N.msm(bases, s)
+    assert np.array_equal(got, O.g1_to_affine(O.msm_g1(bases, s))[0])
+
+
+def test_msm_g2_vs_oracle():
+    n = 2048
+    g2 = O.g2_generator()
+    bases = N.fixed_base_mul(g2, rand_fr(901, n), g2=True)
+    s = rand_fr(902, n)
+    got = N.msm(bases, s, g2=True)
+    assert np.array_equal(got, O.g2_to_affine(O.msm_g2(bases, s))[0])
+
+
+@pytest.mark.parametrize("log_n", [16, 18])
+def test_msm_closed_form(log_n):
+    """Bases k_i G: sum s_i (k_i G) = (sum k_i s_i) G, both scalar kinds."""
+    n = 1 << log_n
+    k = rand_fr(1000 + log_n, n)
+    s = rand_fr(1100 + log_n, n)
+    g1 = O.g1_generator()
+    bases = N.fixed_base_mul(g1, k)
+    m = N.Msm(bases)
+    kk = O.fr_ints(k)
+    for variant in ("uniform", "witness"):
+        if variant == "witness":
+            rng = np.random.default_rng(log_n)
+            pick = rng.random(n) < 0.5
+            bit = rng.random(n) < 0.5
+            s[pick & bit] = O.int_to_words(O.to_mont_int(1, O.R))
+            s[pick & ~bit] = 0
+        tot = sum(a * b for a, b in zip(kk, O.fr_ints(s))) % O.R
+        want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
+        assert np.array_equal(m.run(s), want), variant
+    m.close()
