@@ -97,6 +97,22 @@ int nkr_g16_prove(nkr_g16_ctx *ctx, const void *witness, void *raw_out);
  * order) when the pointers are non-NULL: A*z, B*z, C*z and H. */
 int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h);
 
+/* As nkr_g16_prove with the witness already in device memory (cols Fr). */
+int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out);
+
+/* Stage timing with HIP events on the context's stream.  While enabled,
+ * every prove adds its per-stage device time (ms) to running totals:
+ *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit
+ *   sort (2 schedules), 4 MSM bucket accumulation (5 launches), 5 MSM folding
+ *   and bit planes, 6 plane download + host finish.
+ * counters[0] = proves timed, counters[1] = accumulation launches,
+ * counters[2] = bucket entries (non-zero signed digits) of the witness
+ * schedule, summed over proves (each is used by 3 G1 MSMs and the G2 MSM),
+ * counters[3] = bucket entries of the H schedule (the fifth MSM). */
+#define NKR_G16_STAGES 7
+int nkr_g16_profile(nkr_g16_ctx *ctx, int enable);
+int nkr_g16_profile_read(const nkr_g16_ctx *ctx, double *stage_ms, uint64_t *counters);
+
 /* blind_proof with caller-supplied r, s (Fr, Montgomery).  Host arithmetic. */
 int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
 
@@ -151,6 +167,9 @@ int nkr_dev_free(void *ptr);
 int nkr_memcpy_htod(void *dst_dev, const void *src, uint64_t bytes);
 int nkr_memcpy_dtoh(void *dst, const void *src_dev, uint64_t bytes);
 int nkr_device_synchronize(void);
+/* Plain streaming copy kernel (16-byte vectors): a known-traffic reference
+ * for calibrating hardware byte counters.  bytes must be a multiple of 16. */
+int nkr_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, void *stream);
 
 #ifdef __cplusplus
 }

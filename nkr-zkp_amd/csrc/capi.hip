@@ -80,6 +80,12 @@ __global__ void k_field_op(int op, const uint8_t *a, const uint8_t *b, uint8_t *
     }
 }
 
+__global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 struct nkr_g16_ctx {
@@ -152,6 +158,30 @@ int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void 
     return guarded([&] {
         NKR_REQUIRE(ctx, "null context");
         ctx->prover->prove(witness, raw_out, az, bz, cz, h);
+    });
+}
+
+int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx, "null context");
+        ctx->prover->prove_dev(static_cast<const fr_t *>(witness_dev), raw_out);
+    });
+}
+
+int nkr_g16_profile(nkr_g16_ctx *ctx, int enable)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx, "null context");
+        ctx->prover->set_profiling(enable != 0);
+    });
+}
+
+int nkr_g16_profile_read(const nkr_g16_ctx *ctx, double *stage_ms, uint64_t *counters)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx && stage_ms && counters, "null argument");
+        ctx->prover->profile_read(stage_ms, counters);
     });
 }
 
@@ -454,6 +484,20 @@ int nkr_memcpy_dtoh(void *dst, const void *src, uint64_t bytes)
 int nkr_device_synchronize(void)
 {
     return guarded([&] { NKR_HIP(hipDeviceSynchronize()); });
+}
+
+int nkr_dev_copy(void *dst, const void *src, uint64_t bytes, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(dst && src, "null argument");
+        NKR_REQUIRE(bytes % 16 == 0, "copy size must be a multiple of 16 bytes");
+        const uint64_t n = bytes / 16;
+        if (n == 0) return;
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>(div_up(n, 256), 8192);
+        k_copy16<<<blocks, 256, 0, static_cast<hipStream_t>(stream)>>>(static_cast<const uint4 *>(src),
+                                                                        static_cast<uint4 *>(dst), n);
+        NKR_LAUNCH_CHECK();
+    });
 }
 
 }  // extern "C"

@@ -59,8 +59,9 @@ private:
 // MSM that uses the same scalars (the witness drives four of the five).
 class MsmSchedule {
 public:
-    void build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream);
+    void build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer = nullptr);
     const MsmPlan &plan() const { return plan_; }
+    uint32_t entries() const { return host_total_; }
 
     MsmPlan plan_;
     uint32_t blocks_ = 0;
@@ -70,14 +71,19 @@ public:
     DeviceBuffer<uint32_t> block_base_;    // blocks * B
     DeviceBuffer<uint32_t> max_count_;     // 1
     uint32_t host_max_count_ = 0;
+    uint32_t host_total_ = 0;
 };
+
+// Stage indices used with StageTimer by the MSM code.
+enum MsmStage { kStageSort = 3, kStageAccumulate = 4, kStageFold = 5 };
 
 // Evaluates one MSM given prepared bases and a schedule of the same n and c.
 // Writes the c bit-plane sums U_j (XYZZ) to `planes` (device, c points).
 template <class F>
 class MsmRunner {
 public:
-    void run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream);
+    void run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream,
+             StageTimer *timer = nullptr);
 
 private:
     DeviceBuffer<Xyzz<F>> part_[2];

@@ -282,8 +282,9 @@ void MsmBases<F>::prepare(const Jacobian<F> *jacobian, const uint32_t *perm, con
     NKR_LAUNCH_CHECK();
 }
 
-void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream)
+void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer)
 {
+    if (timer) timer->begin(kStageSort, stream);
     plan_ = plan;
     const uint32_t n = plan.n, B = plan.B;
     blocks_ = std::min<uint32_t>(kSortBlocksMax, div_up(n, 256));
@@ -304,12 +305,15 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
     k_scatter<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, bucket_start_.get(),
                                              block_base_.get(), sorted_.get());
     NKR_LAUNCH_CHECK();
+    if (timer) timer->end(kStageSort, stream);
     NKR_HIP(hipMemcpyAsync(&host_max_count_, max_count_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    NKR_HIP(hipMemcpyAsync(&host_total_, bucket_start_.get() + B, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     NKR_HIP(hipStreamSynchronize(stream));
 }
 
 template <class F>
-void MsmRunner<F>::run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream)
+void MsmRunner<F>::run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream,
+                       StageTimer *timer)
 {
     const MsmPlan &p = sched.plan();
     NKR_REQUIRE(bases.plan().n == p.n && bases.plan().c == p.c, "MSM bases and schedule disagree");
@@ -324,10 +328,15 @@ void MsmRunner<F>::run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<
     // level 0: bases into bucket pieces
     k_scan<<<1, 1024, 0, stream>>>(sched.count_.get(), B, 2, kPieceT0, start_[0].get(), nullptr);
     NKR_LAUNCH_CHECK();
+    if (timer) timer->begin(kStageAccumulate, stream);
     k_accumulate<F><<<div_up(ub0, 256), 256, 0, stream>>>(sched.sorted_.get(), sched.bucket_start_.get(),
                                                           start_[0].get(), B, kPieceT0, bases.data(),
                                                           part_[0].get());
     NKR_LAUNCH_CHECK();
+    if (timer) {
+        timer->end(kStageAccumulate, stream);
+        timer->begin(kStageFold, stream);
+    }
     // folding levels until each bucket has one partial
     uint64_t maxp = (sched.host_max_count_ + kPieceT0 - 1) / kPieceT0;
     uint64_t ub = ub0;
@@ -389,6 +398,7 @@ void MsmRunner<F>::run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<
         k_seg_reduce<F><<<div_up(total, 256), 256, 0, stream>>>(src, in_st, out_st, c, kPieceT1, dst);
         NKR_LAUNCH_CHECK();
     }
+    if (timer) timer->end(kStageFold, stream);
 }
 
 template <class F>

@@ -29,6 +29,12 @@ public:
 
     // raw_out: 576 B.  az/bz/cz/h: optional host outputs (rows Fr, natural order).
     void prove(const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h);
+    // witness already in device memory
+    void prove_dev(const fr_t *witness_dev, void *raw_out);
+
+    static constexpr int kStages = 7;
+    void set_profiling(bool on);
+    void profile_read(double *stage_ms, uint64_t *counters) const;
 
     uint64_t device_bytes() const;
     int device() const { return device_; }
@@ -52,6 +58,11 @@ private:
     DeviceBuffer<Xyzz<fq_t>> planes1_;
     DeviceBuffer<Xyzz<fq2_t>> planes2_;
     void *pinned_witness_ = nullptr;
+
+    void run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h);
+    StageTimer timer_;
+    double stage_ms_[kStages] = {};
+    uint64_t counters_[4] = {};
 };
 
 // blind_proof (protocols/groth16/common/src/protocol.cpp:85-94) with given r, s.

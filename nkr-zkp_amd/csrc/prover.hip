@@ -1,5 +1,6 @@
 #include "prover.h"
 
+#include <chrono>
 #include <cstring>
 
 namespace nkr {
@@ -134,12 +135,40 @@ void Groth16Prover::prove(const void *witness, void *raw_out, void *az, void *bz
 {
     NKR_REQUIRE(witness && raw_out, "null witness or output");
     NKR_HIP(hipSetDevice(device_));
-    hipStream_t s = stream_;
     const size_t fr_bytes = (size_t)cols_ * sizeof(fr_t);
     std::memcpy(pinned_witness_, witness, fr_bytes);
-    NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, s));
+    NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, stream_));
+    run(z_.get(), raw_out, az, bz, cz, h);
+}
 
-    spmv3(views_[0], views_[1], views_[2], z_.get(), pa_.get(), pb_.get(), pc_.get(), s);
+void Groth16Prover::prove_dev(const fr_t *witness_dev, void *raw_out)
+{
+    NKR_REQUIRE(witness_dev && raw_out, "null witness or output");
+    NKR_HIP(hipSetDevice(device_));
+    run(witness_dev, raw_out, nullptr, nullptr, nullptr, nullptr);
+}
+
+void Groth16Prover::set_profiling(bool on)
+{
+    timer_.enable(on);
+    for (double &v : stage_ms_) v = 0;
+    for (uint64_t &v : counters_) v = 0;
+}
+
+void Groth16Prover::profile_read(double *stage_ms, uint64_t *counters) const
+{
+    for (int i = 0; i < kStages; ++i) stage_ms[i] = stage_ms_[i];
+    for (int i = 0; i < 4; ++i) counters[i] = counters_[i];
+}
+
+void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h)
+{
+    hipStream_t s = stream_;
+    StageTimer *t = timer_.enabled() ? &timer_ : nullptr;
+
+    if (t) t->begin(0, s);
+    spmv3(views_[0], views_[1], views_[2], z, pa_.get(), pb_.get(), pc_.get(), s);
+    if (t) t->end(0, s);
     const size_t poly_bytes = (size_t)rows_ * sizeof(fr_t);
     if (az) NKR_HIP(hipMemcpyAsync(az, pa_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
     if (bz) NKR_HIP(hipMemcpyAsync(bz, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
@@ -147,25 +176,36 @@ void Groth16Prover::prove(const void *witness, void *raw_out, void *az, void *bz
 
     // quotient chain, no bit-reversal passes
     fr_t *polys[3] = {pa_.get(), pb_.get(), pc_.get()};
+    if (t) t->begin(1, s);
     for (fr_t *p : polys) domain_->dif(p, true, nullptr, nullptr, nullptr, s);
     for (fr_t *p : polys) domain_->dit(p, false, domain_->coset_rev_scaled(), nullptr, s);
+    if (t) {
+        t->end(1, s);
+        t->begin(2, s);
+    }
     domain_->quotient(pa_.get(), pb_.get(), pc_.get(), pa_.get(), s);
+    if (t) {
+        t->end(2, s);
+        t->begin(1, s);
+    }
     domain_->dif(pa_.get(), true, nullptr, domain_->icoset_rev_scaled(), nullptr, s);
+    if (t) t->end(1, s);
     if (h) {
         domain_->bitrev(pa_.get(), pb_.get(), s);
         NKR_HIP(hipMemcpyAsync(h, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
     }
 
-    // MSMs
-    wsched_.build(z_.get(), wplan_, s);
+    // MSMs: one digit sort of the witness serves ar, bs1, zk and bs2.
+    wsched_.build(z, wplan_, s, t);
     const uint32_t cw = wplan_.c;
-    run1_.run(a_, wsched_, planes1_.get() + 0 * cw, s);
-    run1_.run(b1_, wsched_, planes1_.get() + 1 * cw, s);
-    run1_.run(l_, wsched_, planes1_.get() + 2 * cw, s);
-    run2_.run(b2_, wsched_, planes2_.get(), s);
-    hsched_.build(pa_.get(), hplan_, s);
-    run1_.run(h_, hsched_, planes1_.get() + 3 * cw, s);
+    run1_.run(a_, wsched_, planes1_.get() + 0 * cw, s, t);
+    run1_.run(b1_, wsched_, planes1_.get() + 1 * cw, s, t);
+    run1_.run(l_, wsched_, planes1_.get() + 2 * cw, s, t);
+    run2_.run(b2_, wsched_, planes2_.get(), s, t);
+    hsched_.build(pa_.get(), hplan_, s, t);
+    run1_.run(h_, hsched_, planes1_.get() + 3 * cw, s, t);
 
+    const auto host0 = std::chrono::steady_clock::now();
     std::vector<Xyzz<fq_t>> p1(3 * cw + hplan_.c);
     std::vector<Xyzz<fq2_t>> p2(cw);
     NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
@@ -183,6 +223,15 @@ void Groth16Prover::prove(const void *witness, void *raw_out, void *az, void *bz
     std::memcpy(out + 192, &zk, 96);
     std::memcpy(out + 288, &qz, 96);
     std::memcpy(out + 384, &bs2, 192);
+
+    if (t) {
+        t->collect(stage_ms_);
+        stage_ms_[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
+        counters_[0] += 1;
+        counters_[1] += 5;
+        counters_[2] += wsched_.entries();
+        counters_[3] += hsched_.entries();
+    }
 }
 
 namespace {

@@ -10,6 +10,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace nkr {
 
@@ -88,6 +89,68 @@ public:
 private:
     T *ptr_ = nullptr;
     size_t count_ = 0;
+};
+
+// Device-time accounting per stage with HIP events recorded on the stream
+// that runs the work.  Disabled timers record nothing.
+class StageTimer {
+public:
+    StageTimer() = default;
+    StageTimer(const StageTimer &) = delete;
+    StageTimer &operator=(const StageTimer &) = delete;
+    ~StageTimer()
+    {
+        for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+    }
+    void enable(bool on) { on_ = on; }
+    bool enabled() const { return on_; }
+    void begin(int stage, hipStream_t s)
+    {
+        if (!on_) return;
+        recs_.push_back({stage, take(s), nullptr});
+    }
+    void end(int stage, hipStream_t s)
+    {
+        if (!on_) return;
+        for (auto it = recs_.rbegin(); it != recs_.rend(); ++it)
+            if (it->stage == stage && it->b == nullptr) {
+                it->b = take(s);
+                return;
+            }
+    }
+    // Call after the stream has drained: adds elapsed ms per stage.
+    void collect(double *acc_ms)
+    {
+        for (const Rec &r : recs_) {
+            if (!r.b) continue;
+            float ms = 0.f;
+            NKR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            acc_ms[r.stage] += ms;
+        }
+        recs_.clear();
+        used_ = 0;
+    }
+
+private:
+    struct Rec {
+        int stage;
+        hipEvent_t a, b;
+    };
+    hipEvent_t take(hipStream_t s)
+    {
+        if (used_ == pool_.size()) {
+            hipEvent_t e;
+            NKR_HIP(hipEventCreate(&e));
+            pool_.push_back(e);
+        }
+        hipEvent_t e = pool_[used_++];
+        NKR_HIP(hipEventRecord(e, s));
+        return e;
+    }
+    bool on_ = false;
+    std::vector<hipEvent_t> pool_;
+    size_t used_ = 0;
+    std::vector<Rec> recs_;
 };
 
 inline unsigned ceil_log2(uint64_t n)

@@ -59,6 +59,10 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_prove": [vp, vp, vp],
             "nkr_g16_prove_ex": [vp, vp, vp, vp, vp, vp, vp],
             "nkr_g16_blind": [ctypes.POINTER(Pk), vp, vp, vp, vp],
+            "nkr_g16_prove_dev": [vp, vp, vp],
+            "nkr_g16_profile": [vp, i32],
+            "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
+            "nkr_dev_copy": [vp, vp, u64, vp],
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
