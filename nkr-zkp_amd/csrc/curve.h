@@ -262,6 +262,29 @@ NKR_HD Xyzz<F> xyzz_mul_u256(const Xyzz<F> &p, const uint32_t k[8])
     return acc;
 }
 
+// k*P with a fixed 4-bit window: 14 additions of precomputation, then 4
+// doublings and at most one addition per nibble (host-side blinding).
+template <class F>
+NKR_HD Xyzz<F> xyzz_mul_u256_w4(const Xyzz<F> &p, const uint32_t k[8])
+{
+    Xyzz<F> table[16];
+    table[0] = Xyzz<F>::infinity();
+    table[1] = p;
+    for (int i = 2; i < 16; ++i) table[i] = xyzz_add(table[i - 1], p);
+    Xyzz<F> acc = Xyzz<F>::infinity();
+    bool started = false;
+    for (int nib = 63; nib >= 0; --nib) {
+        if (started)
+            for (int d = 0; d < 4; ++d) acc = xyzz_dbl(acc);
+        const uint32_t v = (k[nib >> 3] >> ((nib & 7) * 4)) & 15u;
+        if (v) {
+            acc = xyzz_add(acc, table[v]);
+            started = true;
+        }
+    }
+    return acc;
+}
+
 using g1_affine = Affine<fq_t>;
 using g2_affine = Affine<fq2_t>;
 using g1_xyzz = Xyzz<fq_t>;

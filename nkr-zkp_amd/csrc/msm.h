@@ -10,14 +10,20 @@
 //  * Scalars are recoded into signed c-bit digits; a counting sort groups the
 //    (window, point, sign) entries by bucket using block-local LDS histograms,
 //    one global atomic per (block, bucket) to reserve ranges, and an LDS
-//    cursor scatter -- no inter-workgroup waiting anywhere.
-//  * Buckets are summed by fixed-size pieces (at most T entries per thread,
-//    mixed affine additions into XYZZ), and the partial sums of each bucket
-//    are folded by further levels of the same segmented reduction, so skewed
-//    digit distributions (witness scalars that are mostly 0 or 1) stay
-//    balanced.
-//  * sum_m m * S_m is formed as c bit-planes U_j = sum_{m has bit j} S_m on the
-//    device; the host combines sum_j 2^j U_j (c doublings) and normalises.
+//    cursor scatter -- no inter-workgroup waiting and no host round trip.
+//  * Several MSMs run as one batch ("jobs"): every stage below is a single
+//    launch over all jobs, so the latency of each reduction level is paid
+//    once per batch, not once per MSM.
+//      1. accumulate: one thread sums up to 64 entries of one bucket
+//         (mixed affine additions into XYZZ);
+//      2. fold: buckets left with more than 256 partial sums (skewed digit
+//         distributions, e.g. witnesses that are mostly 0 or 1) are folded
+//         16:1 until they are not; other buckets are untouched;
+//      3. complete: 16 lanes per bucket, strided partial sums then a 4-step
+//         shuffle tree, give the bucket sums S_m;
+//      4. bit planes: a log-depth tree merges bucket blocks into
+//         U_j = sum over m with bit j of S_m (c points per job).
+//    The host combines sum_j 2^j U_j (c doublings) and normalises.
 // The reference computes the same sums with libff multi_exp<BDLO12>
 // (third_party/.../libff/algebra/scalar_multiplication/multiexp.tcc:165-282);
 // the affine result is unique, so it matches the reference bit for bit.
@@ -31,12 +37,18 @@
 namespace nkr {
 
 struct MsmPlan {
-    uint32_t n = 0;      // number of bases / scalars
-    uint32_t c = 0;      // window width in bits
-    uint32_t W = 0;      // windows (signed digits), c*W >= 255
-    uint32_t B = 0;      // buckets 1..B, B = 2^(c-1)
+    uint32_t n = 0;  // number of bases / scalars
+    uint32_t c = 0;  // window width in bits
+    uint32_t W = 0;  // windows (signed digits), c*W >= 255
+    uint32_t B = 0;  // buckets 1..B, B = 2^(c-1)
     static MsmPlan choose(uint32_t n, uint32_t c_override = 0);
+    uint64_t max_entries() const { return (uint64_t)n * W; }
 };
+
+// Entries per accumulation thread, fold fan-in and the completion limit.
+constexpr uint32_t kMsmPiece = 64;
+constexpr uint32_t kMsmFoldIn = 16;
+constexpr uint32_t kMsmCompleteMax = 256;
 
 // Window-shifted affine bases of one proving-key vector.
 template <class F>
@@ -57,40 +69,50 @@ private:
 
 // Signed-digit recoding and bucket sort of one scalar vector; shared by every
 // MSM that uses the same scalars (the witness drives four of the five).
+// Entirely asynchronous on `stream`.
 class MsmSchedule {
 public:
     void build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer = nullptr);
     const MsmPlan &plan() const { return plan_; }
-    uint32_t entries() const { return host_total_; }
+    uint64_t max_pieces() const { return plan_.max_entries() / kMsmPiece + plan_.B + 1; }
+    // Device counters: [0] entries (non-zero digits), [1] level-0 pieces.
+    const uint32_t *totals() const { return bucket_start_.get() + plan_.B; }
 
     MsmPlan plan_;
     uint32_t blocks_ = 0;
     DeviceBuffer<uint32_t> sorted_;        // entries: sign << 31 | (w * n + i)
     DeviceBuffer<uint32_t> count_;         // per bucket (B)
-    DeviceBuffer<uint32_t> bucket_start_;  // B + 1
+    DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the piece total at [B + 1]
+    DeviceBuffer<uint32_t> piece_start_;   // B + 1
     DeviceBuffer<uint32_t> block_base_;    // blocks * B
-    DeviceBuffer<uint32_t> max_count_;     // 1
-    uint32_t host_max_count_ = 0;
-    uint32_t host_total_ = 0;
 };
 
 // Stage indices used with StageTimer by the MSM code.
 enum MsmStage { kStageSort = 3, kStageAccumulate = 4, kStageFold = 5 };
 
-// Evaluates one MSM given prepared bases and a schedule of the same n and c.
-// Writes the c bit-plane sums U_j (XYZZ) to `planes` (device, c points).
 template <class F>
-class MsmRunner {
+struct MsmJob {
+    const MsmBases<F> *bases;
+    const MsmSchedule *sched;
+    Xyzz<F> *planes;  // device output: c points U_0 .. U_{c-1}
+};
+
+// Runs a batch of MSMs of one group (all with the same window width).
+template <class F>
+class MsmBatch {
 public:
-    void run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream,
-             StageTimer *timer = nullptr);
+    void run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer = nullptr);
 
 private:
-    DeviceBuffer<Xyzz<F>> part_[2];
-    DeviceBuffer<uint32_t> start_[2];
-    DeviceBuffer<Xyzz<F>> sums_;
-    DeviceBuffer<uint32_t> plane_start_;
-    std::vector<uint32_t> plane_levels_;
+    void layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream);
+    std::vector<const void *> key_;
+    uint32_t fold_levels_ = 0;
+    uint64_t pool_per_job_ = 0, thread_total_ = 0;
+    std::vector<uint32_t> thread_base_;
+    DeviceBuffer<uint8_t> jobs_dev_;
+    DeviceBuffer<uint32_t> thread_base_dev_;
+    DeviceBuffer<Xyzz<F>> pool_, sums_, tree_[2];
+    DeviceBuffer<uint32_t> reg_start_, reg_count_, fold_start_;
 };
 
 // Host: sum_j 2^j U_j, then the reference's affine form (Z = 1, or (0, 1, 0)).
@@ -108,7 +130,7 @@ public:
 private:
     MsmBases<F> bases_;
     MsmSchedule sched_;
-    MsmRunner<F> runner_;
+    MsmBatch<F> batch_;
     DeviceBuffer<Xyzz<F>> planes_;
 };
 

@@ -2,16 +2,31 @@
 #include "msm.h"
 
 #include <algorithm>
+#include <cstring>
 
 namespace nkr {
 
 namespace {
 
 constexpr uint32_t kEntryMask = 0x7fffffffu;
-constexpr uint32_t kPieceT0 = 16;   // entries per thread when summing bases into buckets
-constexpr uint32_t kPieceT1 = 8;    // partials per thread in the folding levels
-constexpr uint32_t kPlaneT = 16;    // buckets per thread when forming bit planes
 constexpr uint32_t kSortBlocksMax = 1024;
+
+template <class F>
+struct JobDev {
+    const Affine<F> *bases;
+    const uint32_t *sorted;
+    const uint32_t *bucket_start;  // B + 1
+    const uint32_t *piece_start;   // B + 1
+    const uint32_t *piece_max;     // largest level-0 piece count of any bucket
+    Xyzz<F> *pool;                 // partial sums (level 0 region, then fold levels)
+    uint32_t *reg_start;           // B: where bucket k's partials live in `pool`
+    uint32_t *reg_count;           // B
+    uint32_t *fold_start;          // B + 1
+    Xyzz<F> *sums;                 // B: S_{k+1}
+    Xyzz<F> *tree[2];              // B each
+    Xyzz<F> *planes;               // c
+    uint32_t B, c;
+};
 
 // ------------------------------------------------------------ digit recoding
 // Signed c-bit digits of an integer scalar (< 2^254): d_w in [-2^(c-1), 2^(c-1)],
@@ -100,20 +115,22 @@ __global__ __launch_bounds__(256) void k_scatter(const fr_t *__restrict__ scalar
     }
 }
 
-// Exclusive prefix over `len` values into out[0..len]; optional maximum.
+// Exclusive prefix of `len` values into out[0..len] within one workgroup.
 //   mode 0: v[k] = src[k]
-//   mode 1: v[k] = ceil((src[k+1] - src[k]) / T)     (next folding level)
-//   mode 2: v[k] = ceil(src[k] / T)                   (pieces of a bucket)
-__global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ src, uint32_t len, uint32_t mode,
-                                               uint32_t T, uint32_t *__restrict__ out,
-                                               uint32_t *__restrict__ max_out)
+//   mode 2: v[k] = ceil(src[k] / T)
+//   mode 3: v[k] = src[k] > lim ? ceil(src[k] / T) : 0
+__device__ void block_scan(const uint32_t *__restrict__ src, uint32_t len, uint32_t mode, uint32_t T, uint32_t lim,
+                           uint32_t *__restrict__ out, uint32_t *__restrict__ max_out = nullptr)
 {
     __shared__ uint32_t part[1024];
-    __shared__ uint32_t pmax[1024];
+    __shared__ uint32_t pmax;
+    if (threadIdx.x == 0) pmax = 0;
+    __syncthreads();
     auto value = [&](uint32_t k) -> uint32_t {
-        if (mode == 0) return src[k];
-        if (mode == 1) return (src[k + 1] - src[k] + T - 1) / T;
-        return (src[k] + T - 1) / T;
+        const uint32_t x = src[k];
+        if (mode == 0) return x;
+        if (mode == 2) return (x + T - 1) / T;
+        return x > lim ? (x + T - 1) / T : 0u;
     };
     const uint32_t per = (len + blockDim.x - 1) / blockDim.x;
     const uint32_t beg = min(len, threadIdx.x * per);
@@ -125,15 +142,12 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ src,
         mx = max(mx, v);
     }
     part[threadIdx.x] = sum;
-    pmax[threadIdx.x] = mx;
+    if (max_out && mx) atomicMax(&pmax, mx);
     __syncthreads();
-    // Hillis-Steele inclusive scan of the per-thread sums.
     for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
         const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        const uint32_t m = threadIdx.x >= off ? pmax[threadIdx.x - off] : 0u;
         __syncthreads();
         part[threadIdx.x] += t;
-        pmax[threadIdx.x] = max(pmax[threadIdx.x], m);
         __syncthreads();
     }
     uint32_t run = part[threadIdx.x] - sum;
@@ -141,10 +155,19 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t *__restrict__ src,
         out[k] = run;
         run += value(k);
     }
-    if (threadIdx.x == blockDim.x - 1) {
-        out[len] = part[threadIdx.x];
-        if (max_out) *max_out = pmax[threadIdx.x];
-    }
+    if (threadIdx.x == blockDim.x - 1) out[len] = part[threadIdx.x];
+    if (max_out && threadIdx.x == 0) *max_out = pmax;
+    __syncthreads();
+}
+
+// Bucket starts (mode 0 over counts) and level-0 piece starts (ceil(count/64)).
+__global__ __launch_bounds__(1024) void k_schedule_scan(const uint32_t *__restrict__ count, uint32_t B,
+                                                        uint32_t *__restrict__ bucket_start,
+                                                        uint32_t *__restrict__ piece_start)
+{
+    block_scan(count, B, 0, 1, 0, bucket_start);
+    block_scan(count, B, 2, kMsmPiece, 0, piece_start, bucket_start + B + 2);
+    if (threadIdx.x == 0) bucket_start[B + 1] = piece_start[B];
 }
 
 // Last index k in [0, len) with starts[k] <= p (starts[0] <= p < starts[len]).
@@ -161,19 +184,33 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t *starts, uint32_
     return lo;
 }
 
-// Level 0: one thread sums up to T signed bases of one bucket.
 template <class F>
-__global__ __launch_bounds__(256) void k_accumulate(const uint32_t *__restrict__ sorted,
-                                                    const uint32_t *__restrict__ bucket_start,
-                                                    const uint32_t *__restrict__ piece_start, uint32_t B, uint32_t T,
-                                                    const Affine<F> *__restrict__ bases, Xyzz<F> *__restrict__ out)
+__global__ __launch_bounds__(256) void k_regions_init(const JobDev<F> *__restrict__ jobs)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= piece_start[B]) return;
-    const uint32_t k = find_segment(piece_start, B, p);
-    const uint32_t q = p - piece_start[k];
-    const uint32_t beg = bucket_start[k] + q * T;
-    const uint32_t end = min(beg + T, bucket_start[k + 1]);
+    const JobDev<F> &jb = jobs[blockIdx.y];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= jb.B) return;
+    jb.reg_start[k] = jb.piece_start[k];
+    jb.reg_count[k] = jb.piece_start[k + 1] - jb.piece_start[k];
+}
+
+// Level 0 over all jobs: one thread sums up to kMsmPiece signed bases of one bucket.
+template <class F>
+__global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ thread_base, uint32_t J)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= thread_base[J]) return;
+    const uint32_t j = find_segment(thread_base, J, g);
+    const JobDev<F> &jb = jobs[j];
+    const uint32_t p = g - thread_base[j];
+    if (p >= jb.piece_start[jb.B]) return;
+    const uint32_t k = find_segment(jb.piece_start, jb.B, p);
+    const uint32_t q = p - jb.piece_start[k];
+    const uint32_t beg = jb.bucket_start[k] + q * kMsmPiece;
+    const uint32_t end = min(beg + kMsmPiece, jb.bucket_start[k + 1]);
+    const Affine<F> *__restrict__ bases = jb.bases;
+    const uint32_t *__restrict__ sorted = jb.sorted;
     Xyzz<F> acc = Xyzz<F>::infinity();
     for (uint32_t e = beg; e < end; ++e) {
         const uint32_t v = sorted[e];
@@ -182,48 +219,133 @@ __global__ __launch_bounds__(256) void k_accumulate(const uint32_t *__restrict__
         const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
         acc = xyzz_add_affine(acc, P.x, y);
     }
-    out[p] = acc;
+    jb.pool[p] = acc;
 }
 
-// Folding level: one thread sums up to T consecutive partials of one segment.
+// Fold level: buckets with more than kMsmCompleteMax partials get
+// ceil(count / kMsmFoldIn) new partials written at pool[level_base ...].
 template <class F>
-__global__ __launch_bounds__(256) void k_seg_reduce(const Xyzz<F> *__restrict__ in,
-                                                    const uint32_t *__restrict__ in_start,
-                                                    const uint32_t *__restrict__ out_start, uint32_t S, uint32_t T,
-                                                    Xyzz<F> *__restrict__ out)
+__global__ __launch_bounds__(1024) void k_fold_scan(const JobDev<F> *__restrict__ jobs)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= out_start[S]) return;
-    const uint32_t k = find_segment(out_start, S, p);
-    const uint32_t q = p - out_start[k];
-    const uint32_t beg = in_start[k] + q * T;
-    const uint32_t end = min(beg + T, in_start[k + 1]);
-    Xyzz<F> acc = in[beg];
-    for (uint32_t e = beg + 1; e < end; ++e) acc = xyzz_add(acc, in[e]);
-    out[p] = acc;
+    const JobDev<F> &jb = jobs[blockIdx.x];
+    if (*jb.piece_max <= kMsmCompleteMax) return;  // no skewed bucket: nothing to fold
+    block_scan(jb.reg_count, jb.B, 3, kMsmFoldIn, kMsmCompleteMax, jb.fold_start);
 }
 
-// Bit planes, first level: plane j gathers bucket sums S_m for m with bit j.
 template <class F>
-__global__ __launch_bounds__(256) void k_planes(const Xyzz<F> *__restrict__ part, const uint32_t *__restrict__ start,
-                                                uint32_t c, uint32_t T, const uint32_t *__restrict__ pstart,
-                                                Xyzz<F> *__restrict__ out)
+__global__ __launch_bounds__(256) void k_fold(const JobDev<F> *__restrict__ jobs, uint32_t level_base)
 {
+    const JobDev<F> &jb = jobs[blockIdx.y];
+    if (*jb.piece_max <= kMsmCompleteMax) return;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pstart[c]) return;
-    const uint32_t j = find_segment(pstart, c, p);
-    const uint32_t r = p - pstart[j];
-    const uint32_t cnt = (j == c - 1) ? 1u : (1u << (c - 2));
-    Xyzz<F> acc = Xyzz<F>::infinity();
-    for (uint32_t u = 0; u < T; ++u) {
-        const uint32_t t = r * T + u;
-        if (t >= cnt) break;
-        const uint32_t m = (j == c - 1) ? (1u << (c - 1))
-                                        : (((t >> j) << (j + 1)) | (1u << j) | (t & ((1u << j) - 1u)));
-        const uint32_t k = m - 1;
-        if (start[k + 1] > start[k]) acc = xyzz_add(acc, part[start[k]]);
+    if (p >= jb.fold_start[jb.B]) return;
+    const uint32_t k = find_segment(jb.fold_start, jb.B, p);
+    const uint32_t q = p - jb.fold_start[k];
+    const uint32_t beg = jb.reg_start[k] + q * kMsmFoldIn;
+    const uint32_t end = min(beg + kMsmFoldIn, jb.reg_start[k] + jb.reg_count[k]);
+    Xyzz<F> acc = jb.pool[beg];
+    for (uint32_t e = beg + 1; e < end; ++e) acc = xyzz_add(acc, jb.pool[e]);
+    jb.pool[level_base + p] = acc;
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_fold_retarget(const JobDev<F> *__restrict__ jobs, uint32_t level_base)
+{
+    const JobDev<F> &jb = jobs[blockIdx.y];
+    if (*jb.piece_max <= kMsmCompleteMax) return;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= jb.B) return;
+    if (jb.reg_count[k] > kMsmCompleteMax) {
+        jb.reg_start[k] = level_base + jb.fold_start[k];
+        jb.reg_count[k] = jb.fold_start[k + 1] - jb.fold_start[k];
     }
-    out[p] = acc;
+}
+
+template <class F>
+__device__ __forceinline__ Xyzz<F> shfl_xor_point(const Xyzz<F> &p, int mask)
+{
+    Xyzz<F> r;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&p);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(Xyzz<F>) / 4); ++i) dst[i] = (uint32_t)__shfl_xor((int)src[i], mask, 16);
+    return r;
+}
+
+// Bucket completion: 16 lanes per bucket (<= kMsmCompleteMax partials).
+template <class F>
+__global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ jobs)
+{
+    const JobDev<F> &jb = jobs[blockIdx.y];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = gid >> 4;
+    const uint32_t lane = gid & 15u;
+    Xyzz<F> acc = Xyzz<F>::infinity();
+    if (k < jb.B) {
+        const uint32_t base = jb.reg_start[k];
+        const uint32_t cnt = jb.reg_count[k];
+        for (uint32_t i = lane; i < cnt; i += 16) acc = xyzz_add(acc, jb.pool[base + i]);
+    }
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, off));
+    if (k < jb.B && lane == 0) jb.sums[k] = acc;
+}
+
+// Bit-plane tree, level t -> t+1.  A level-t block covers buckets
+// [b*2^t, (b+1)*2^t) and holds P_0..P_{t-1} (sums of its buckets with that
+// bit set) and T (its total).  Level 0 reads S_m (S_0 = infinity) from sums.
+// The last level writes U_j = P_j, and U_{c-1} = S_B, to the planes.
+template <class F>
+__global__ __launch_bounds__(256) void k_tree(const JobDev<F> *__restrict__ jobs, uint32_t t)
+{
+    const JobDev<F> &jb = jobs[blockIdx.y];
+    if (t + 1 > jb.c - 1) return;
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (jb.B >> (t + 1)) * (t + 2)) return;
+    tree_merge(jb, t, idx);
+}
+
+// One merge of the bit-plane tree (level t -> t+1) for output index idx.
+template <class F>
+__device__ __forceinline__ void tree_merge(const JobDev<F> &jb, uint32_t t, uint32_t idx)
+{
+    const uint32_t E = t + 2;
+    const uint32_t b = idx / E, e = idx % E;
+    const bool last = (t + 1 == jb.c - 1);
+    auto in = [&](uint32_t blk, uint32_t ent) -> Xyzz<F> {
+        if (t == 0) return blk == 0 ? Xyzz<F>::infinity() : jb.sums[blk - 1];
+        return jb.tree[t & 1][(size_t)blk * (t + 1) + ent];
+    };
+    Xyzz<F> out;
+    if (e < t) {
+        out = xyzz_add(in(2 * b, e), in(2 * b + 1, e));
+    } else if (e == t) {
+        out = in(2 * b + 1, t);
+    } else {
+        if (last) {
+            jb.planes[jb.c - 1] = jb.sums[jb.B - 1];
+            return;
+        }
+        out = xyzz_add(in(2 * b, t), in(2 * b + 1, t));
+    }
+    if (last)
+        jb.planes[e] = out;
+    else
+        jb.tree[(t + 1) & 1][(size_t)b * E + e] = out;
+}
+
+// The remaining tree levels t0 .. c-2 of one job in one workgroup (blockIdx.x
+// = job), with a barrier between levels instead of a launch.
+template <class F>
+__global__ __launch_bounds__(1024) void k_tree_tail(const JobDev<F> *__restrict__ jobs, uint32_t t0)
+{
+    const JobDev<F> &jb = jobs[blockIdx.x];
+    for (uint32_t t = t0; t + 1 <= jb.c - 1; ++t) {
+        const uint32_t work = (jb.B >> (t + 1)) * (t + 2);
+        for (uint32_t idx = threadIdx.x; idx < work; idx += blockDim.x) tree_merge(jb, t, idx);
+        __threadfence_block();
+        __syncthreads();
+    }
 }
 
 // Base preparation: normalise the reference-layout Jacobian point and store
@@ -284,118 +406,149 @@ void MsmBases<F>::prepare(const Jacobian<F> *jacobian, const uint32_t *perm, con
 
 void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer)
 {
-    if (timer) timer->begin(kStageSort, stream);
     plan_ = plan;
     const uint32_t n = plan.n, B = plan.B;
     blocks_ = std::min<uint32_t>(kSortBlocksMax, div_up(n, 256));
     const uint32_t per_block = div_up(n, blocks_);
     sorted_.ensure((size_t)n * plan.W);
     count_.ensure(B);
-    bucket_start_.ensure(B + 1);
+    bucket_start_.ensure(B + 3);  // + total pieces, + largest piece count of a bucket
+    piece_start_.ensure(B + 1);
     block_base_.ensure((size_t)blocks_ * B);
-    max_count_.ensure(1);
     const size_t lds = (size_t)B * sizeof(uint32_t);
+    if (timer) timer->begin(kStageSort, stream);
     k_zero_u32<<<div_up(B, 256), 256, 0, stream>>>(count_.get(), B);
     NKR_LAUNCH_CHECK();
     k_hist<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, count_.get(),
                                           block_base_.get());
     NKR_LAUNCH_CHECK();
-    k_scan<<<1, 1024, 0, stream>>>(count_.get(), B, 0, 1, bucket_start_.get(), max_count_.get());
+    k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, bucket_start_.get(), piece_start_.get());
     NKR_LAUNCH_CHECK();
     k_scatter<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, bucket_start_.get(),
                                              block_base_.get(), sorted_.get());
     NKR_LAUNCH_CHECK();
     if (timer) timer->end(kStageSort, stream);
-    NKR_HIP(hipMemcpyAsync(&host_max_count_, max_count_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    NKR_HIP(hipMemcpyAsync(&host_total_, bucket_start_.get() + B, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    NKR_HIP(hipStreamSynchronize(stream));
 }
 
 template <class F>
-void MsmRunner<F>::run(const MsmBases<F> &bases, const MsmSchedule &sched, Xyzz<F> *planes, hipStream_t stream,
-                       StageTimer *timer)
+void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
 {
-    const MsmPlan &p = sched.plan();
-    NKR_REQUIRE(bases.plan().n == p.n && bases.plan().c == p.c, "MSM bases and schedule disagree");
-    const uint32_t B = p.B;
-    const uint64_t ub0 = (uint64_t)p.n * p.W / kPieceT0 + B + 1;
-    const uint64_t ub1 = ub0 / kPieceT1 + B + 1;
-    part_[0].ensure(ub0);
-    part_[1].ensure(ub1);
-    start_[0].ensure(B + 1);
-    start_[1].ensure(B + 1);
+    std::vector<const void *> key;
+    for (const auto &jb : jobs) {
+        key.push_back(jb.bases->data());
+        key.push_back(jb.sched->sorted_.get());
+        key.push_back(jb.sched->bucket_start_.get());
+        key.push_back(jb.sched->piece_start_.get());
+        key.push_back(jb.planes);
+    }
+    if (key == key_) return;
+    const uint32_t J = (uint32_t)jobs.size();
+    uint32_t Bmax = 0;
+    uint64_t pmax = 0;
+    for (const auto &jb : jobs) {
+        const MsmPlan &p = jb.sched->plan();
+        NKR_REQUIRE(jb.bases->plan().n == p.n && jb.bases->plan().c == p.c, "MSM bases and schedule disagree");
+        Bmax = std::max(Bmax, p.B);
+        pmax = std::max<uint64_t>(pmax, jb.sched->max_pieces());
+    }
+    // pool per job: level-0 pieces, then each fold level's worst case
+    uint64_t cap = pmax, total = pmax, cnt = pmax;
+    fold_levels_ = 0;
+    while (cnt > kMsmCompleteMax) {
+        cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
+        cap = cap / kMsmFoldIn + Bmax + 1;
+        total += cap;
+        ++fold_levels_;
+    }
+    NKR_REQUIRE(total < (1ull << 32), "MSM partial pool exceeds 32-bit indexing");
+    pool_per_job_ = total;
+    pool_.allocate(pool_per_job_ * J);
+    sums_.allocate((size_t)Bmax * J);
+    tree_[0].allocate((size_t)Bmax * J);
+    tree_[1].allocate((size_t)Bmax * J);
+    reg_start_.allocate((size_t)Bmax * J);
+    reg_count_.allocate((size_t)Bmax * J);
+    fold_start_.allocate((size_t)(Bmax + 1) * J);
 
-    // level 0: bases into bucket pieces
-    k_scan<<<1, 1024, 0, stream>>>(sched.count_.get(), B, 2, kPieceT0, start_[0].get(), nullptr);
+    std::vector<JobDev<F>> host(J);
+    thread_base_.assign(J + 1, 0);
+    for (uint32_t j = 0; j < J; ++j) {
+        const MsmJob<F> &jb = jobs[j];
+        JobDev<F> &d = host[j];
+        d.bases = jb.bases->data();
+        d.sorted = jb.sched->sorted_.get();
+        d.bucket_start = jb.sched->bucket_start_.get();
+        d.piece_start = jb.sched->piece_start_.get();
+        d.piece_max = jb.sched->bucket_start_.get() + jb.sched->plan().B + 2;
+        d.pool = pool_.get() + pool_per_job_ * j;
+        d.reg_start = reg_start_.get() + (size_t)Bmax * j;
+        d.reg_count = reg_count_.get() + (size_t)Bmax * j;
+        d.fold_start = fold_start_.get() + (size_t)(Bmax + 1) * j;
+        d.sums = sums_.get() + (size_t)Bmax * j;
+        d.tree[0] = tree_[0].get() + (size_t)Bmax * j;
+        d.tree[1] = tree_[1].get() + (size_t)Bmax * j;
+        d.planes = jb.planes;
+        d.B = jb.sched->plan().B;
+        d.c = jb.sched->plan().c;
+        thread_base_[j + 1] = thread_base_[j] + (uint32_t)jb.sched->max_pieces();
+    }
+    thread_total_ = thread_base_[J];
+    jobs_dev_.allocate(sizeof(JobDev<F>) * J);
+    thread_base_dev_.allocate(J + 1);
+    NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host.data(), sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
+    NKR_HIP(hipMemcpyAsync(thread_base_dev_.get(), thread_base_.data(), (J + 1) * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, stream));
+    NKR_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
+    key_ = key;
+}
+
+template <class F>
+void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer)
+{
+    if (jobs.empty()) return;
+    layout(jobs, stream);
+    const uint32_t J = (uint32_t)jobs.size();
+    uint32_t Bmax = 0, cmax = 0;
+    for (const auto &jb : jobs) {
+        Bmax = std::max(Bmax, jb.sched->plan().B);
+        cmax = std::max(cmax, jb.sched->plan().c);
+    }
+    const JobDev<F> *jd = reinterpret_cast<const JobDev<F> *>(jobs_dev_.get());
+
+    k_regions_init<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd);
     NKR_LAUNCH_CHECK();
     if (timer) timer->begin(kStageAccumulate, stream);
-    k_accumulate<F><<<div_up(ub0, 256), 256, 0, stream>>>(sched.sorted_.get(), sched.bucket_start_.get(),
-                                                          start_[0].get(), B, kPieceT0, bases.data(),
-                                                          part_[0].get());
+    k_accumulate<F><<<div_up(thread_total_, 256), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
     NKR_LAUNCH_CHECK();
     if (timer) {
         timer->end(kStageAccumulate, stream);
         timer->begin(kStageFold, stream);
     }
-    // folding levels until each bucket has one partial
-    uint64_t maxp = (sched.host_max_count_ + kPieceT0 - 1) / kPieceT0;
-    uint64_t ub = ub0;
-    int cur = 0;
-    while (maxp > 1) {
-        const int nxt = cur ^ 1;
-        const uint64_t ubn = ub / kPieceT1 + B + 1;
-        NKR_REQUIRE(ubn <= part_[nxt].size(), "MSM folding buffer too small");
-        k_scan<<<1, 1024, 0, stream>>>(start_[cur].get(), B, 1, kPieceT1, start_[nxt].get(), nullptr);
+    uint64_t level_base = 0, cap = 0;
+    for (const auto &jb : jobs) cap = std::max<uint64_t>(cap, jb.sched->max_pieces());
+    level_base = cap;
+    for (uint32_t L = 0; L < fold_levels_; ++L) {
+        const uint64_t out_cap = cap / kMsmFoldIn + Bmax + 1;
+        k_fold_scan<F><<<J, 1024, 0, stream>>>(jd);
         NKR_LAUNCH_CHECK();
-        k_seg_reduce<F><<<div_up(ubn, 256), 256, 0, stream>>>(part_[cur].get(), start_[cur].get(),
-                                                              start_[nxt].get(), B, kPieceT1, part_[nxt].get());
+        k_fold<F><<<dim3(div_up(out_cap, 256), J), 256, 0, stream>>>(jd, (uint32_t)level_base);
         NKR_LAUNCH_CHECK();
-        cur = nxt;
-        ub = ubn;
-        maxp = (maxp + kPieceT1 - 1) / kPieceT1;
+        k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, (uint32_t)level_base);
+        NKR_LAUNCH_CHECK();
+        level_base += out_cap;
+        cap = out_cap;
     }
-
-    // bit planes U_j, j < c: static segment layout, folded to one point each
-    const uint32_t c = p.c;
-    if (plane_levels_.empty() || plane_levels_[0] != c) {
-        // layout: [c, levels, then (c+1) offsets per level]
-        std::vector<std::vector<uint32_t>> lv;
-        std::vector<uint32_t> cnt(c);
-        for (uint32_t j = 0; j < c; ++j) cnt[j] = (j == c - 1) ? 1u : (1u << (c - 2));
-        std::vector<uint32_t> pieces(c);
-        for (uint32_t j = 0; j < c; ++j) pieces[j] = (cnt[j] + kPlaneT - 1) / kPlaneT;
-        for (;;) {
-            std::vector<uint32_t> st(c + 1, 0);
-            for (uint32_t j = 0; j < c; ++j) st[j + 1] = st[j] + pieces[j];
-            lv.push_back(st);
-            if (*std::max_element(pieces.begin(), pieces.end()) <= 1) break;
-            for (uint32_t j = 0; j < c; ++j) pieces[j] = (pieces[j] + kPieceT1 - 1) / kPieceT1;
-        }
-        plane_levels_.assign({c, (uint32_t)lv.size()});
-        for (auto &st : lv) plane_levels_.insert(plane_levels_.end(), st.begin(), st.end());
-        plane_start_.allocate(plane_levels_.size());
-        NKR_HIP(hipMemcpyAsync(plane_start_.get(), plane_levels_.data(), plane_levels_.size() * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, stream));
-        sums_.allocate(2 * (size_t)lv[0][c]);
-    }
-    const uint32_t levels = plane_levels_[1];
-    const uint32_t *dev_levels = plane_start_.get() + 2;
-    const uint32_t *host_levels = plane_levels_.data() + 2;
-    Xyzz<F> *bufs[2] = {sums_.get(), sums_.get() + sums_.size() / 2};
-    {
-        Xyzz<F> *dst = (levels == 1) ? planes : bufs[0];
-        const uint32_t total = host_levels[c];
-        k_planes<F><<<div_up(total, 256), 256, 0, stream>>>(part_[cur].get(), start_[cur].get(), c, kPlaneT,
-                                                            dev_levels, dst);
+    k_complete<F><<<dim3(div_up((uint64_t)Bmax * 16, 256), J), 256, 0, stream>>>(jd);
+    NKR_LAUNCH_CHECK();
+    uint32_t t = 0;
+    for (; t + 1 <= cmax - 1; ++t) {
+        const uint64_t work = (uint64_t)(Bmax >> (t + 1)) * (t + 2);
+        if (work <= 1024) break;  // the rest fits one workgroup per job
+        k_tree<F><<<dim3(div_up(work, 256), J), 256, 0, stream>>>(jd, t);
         NKR_LAUNCH_CHECK();
     }
-    for (uint32_t L = 1; L < levels; ++L) {
-        const uint32_t *in_st = dev_levels + (size_t)(L - 1) * (c + 1);
-        const uint32_t *out_st = dev_levels + (size_t)L * (c + 1);
-        const uint32_t total = host_levels[(size_t)L * (c + 1) + c];
-        Xyzz<F> *src = bufs[(L - 1) & 1];
-        Xyzz<F> *dst = (L == levels - 1) ? planes : bufs[L & 1];
-        k_seg_reduce<F><<<div_up(total, 256), 256, 0, stream>>>(src, in_st, out_st, c, kPieceT1, dst);
+    if (t + 1 <= cmax - 1) {
+        k_tree_tail<F><<<J, 1024, 0, stream>>>(jd, t);
         NKR_LAUNCH_CHECK();
     }
     if (timer) timer->end(kStageFold, stream);
@@ -425,7 +578,7 @@ Jacobian<F> MsmContext<F>::run(const fr_t *scalars_dev, hipStream_t stream)
 {
     const MsmPlan &plan = bases_.plan();
     sched_.build(scalars_dev, plan, stream);
-    runner_.run(bases_, sched_, planes_.get(), stream);
+    batch_.run({MsmJob<F>{&bases_, &sched_, planes_.get()}}, stream);
     std::vector<Xyzz<F>> host(plan.c);
     NKR_HIP(hipMemcpyAsync(host.data(), planes_.get(), plan.c * sizeof(Xyzz<F>), hipMemcpyDeviceToHost, stream));
     NKR_HIP(hipStreamSynchronize(stream));
@@ -434,8 +587,8 @@ Jacobian<F> MsmContext<F>::run(const fr_t *scalars_dev, hipStream_t stream)
 
 template class MsmBases<fq_t>;
 template class MsmBases<fq2_t>;
-template class MsmRunner<fq_t>;
-template class MsmRunner<fq2_t>;
+template class MsmBatch<fq_t>;
+template class MsmBatch<fq2_t>;
 template class MsmContext<fq_t>;
 template class MsmContext<fq2_t>;
 template Jacobian<fq_t> msm_finish(const Xyzz<fq_t> *, uint32_t);

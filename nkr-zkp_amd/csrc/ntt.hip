@@ -38,6 +38,52 @@ __device__ __forceinline__ fr_t tw_pow(const PassArgs &a, uint32_t e)
     return fp_mul(a.tw_hi[hi], a.tw_lo[lo]);
 }
 
+// One round of R consecutive radix-2 stages (t .. t+R-1) on groups of 2^R
+// elements held in registers.  A group fixes every index bit except the R
+// bits [np-t-R, np-t) that the round's butterflies pair.  The twiddle of the
+// pair (u, u + 2^(R-1-k)) at stage s = t + k is root^((j mod 2^(np-1-s)) * 2^s)
+// for its lower index j, as in the radix-2 DIF recursion.
+template <bool DIT, int R>
+__device__ __forceinline__ void ntt_round(fr_t *smem, const PassArgs &a, uint32_t t, uint32_t cols,
+                                          uint32_t root_shift)
+{
+    constexpr uint32_t G = 1u << R;
+    const uint32_t low = a.np - t - R;
+    const uint32_t groups = (1u << (a.np - R)) << a.log_cols;
+    for (uint32_t gi = threadIdx.x; gi < groups; gi += blockDim.x) {
+        const uint32_t c = gi & (cols - 1);
+        const uint32_t q = gi >> a.log_cols;
+        const uint32_t lo_part = q & ((1u << low) - 1u);
+        const uint32_t j0 = ((q >> low) << (a.np - t)) | lo_part;
+        fr_t x[G];
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) x[u] = smem[(j0 + (u << low)) * cols + c];
+#pragma unroll
+        for (int kk = 0; kk < R; ++kk) {
+            const int k = DIT ? (R - 1 - kk) : kk;
+            const uint32_t s = t + (uint32_t)k;
+            const uint32_t hs = 1u << (R - 1 - k);
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                if (u & hs) continue;
+                const uint32_t e = (lo_part + ((u & (hs - 1u)) << low)) << s;
+                const fr_t w = a.roots[e << root_shift];
+                const fr_t X = x[u], Y = x[u + hs];
+                if (!DIT) {
+                    x[u] = fp_add(X, Y);
+                    x[u + hs] = fp_mul(fp_sub(X, Y), w);
+                } else {
+                    const fr_t wy = fp_mul(Y, w);
+                    x[u] = fp_add(X, wy);
+                    x[u + hs] = fp_sub(X, wy);
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) smem[(j0 + (u << low)) * cols + c] = x[u];
+    }
+}
+
 template <bool DIT>
 __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs a)
 {
@@ -72,32 +118,22 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs a)
     }
     __syncthreads();
 
-    const uint32_t half = total >> 1;
+    // Sub-transform in rounds of up to 3 radix-2 stages: each thread keeps a
+    // group of 8 elements in registers for the whole round, so LDS is
+    // visited once per 3 stages.  DIF rounds run stages t..t+r-1, DIT rounds
+    // the same stages in reverse (its transpose).
     const uint32_t root_shift = a.log_roots - a.np;
-    for (uint32_t step = 0; step < a.np; ++step) {
-        const uint32_t t = DIT ? (a.np - 1 - step) : step;
-        const uint32_t log_h = a.np - 1 - t;
-        const uint32_t h = 1u << log_h;
-        for (uint32_t b = threadIdx.x; b < half; b += blockDim.x) {
-            const uint32_t c = b & (cols - 1);
-            const uint32_t q = b >> a.log_cols;
-            const uint32_t jl = ((q >> log_h) << (log_h + 1)) | (q & (h - 1));
-            const uint32_t jh = jl + h;
-            const uint32_t e = (q & (h - 1)) << t;
-            const fr_t x = smem[jl * cols + c];
-            const fr_t y = smem[jh * cols + c];
-            if (!DIT) {
-                fr_t d = fp_sub(x, y);
-                if (e) d = fp_mul(d, a.roots[e << root_shift]);
-                smem[jl * cols + c] = fp_add(x, y);
-                smem[jh * cols + c] = d;
-            } else {
-                const fr_t wy = e ? fp_mul(y, a.roots[e << root_shift]) : y;
-                smem[jl * cols + c] = fp_add(x, wy);
-                smem[jh * cols + c] = fp_sub(x, wy);
-            }
-        }
+    for (uint32_t done = 0; done < a.np;) {
+        const uint32_t r = min(3u, a.np - done);
+        const uint32_t t = DIT ? (a.np - done - r) : done;
+        if (r == 3)
+            ntt_round<DIT, 3>(smem, a, t, cols, root_shift);
+        else if (r == 2)
+            ntt_round<DIT, 2>(smem, a, t, cols, root_shift);
+        else
+            ntt_round<DIT, 1>(smem, a, t, cols, root_shift);
         __syncthreads();
+        done += r;
     }
 
     for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {

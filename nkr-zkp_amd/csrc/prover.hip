@@ -60,6 +60,9 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     log_n_ = ceil_log2(rows_);
     NKR_HIP(hipSetDevice(device_));
     NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    NKR_HIP(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
+    NKR_HIP(hipEventCreateWithFlags(&ev_side_done_, hipEventDisableTiming));
 
     const uint32_t *rps[3] = {c.a_row_ptr, c.b_row_ptr, c.c_row_ptr};
     const uint32_t *cis[3] = {c.a_col_idx, c.b_col_idx, c.c_col_idx};
@@ -116,11 +119,17 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
 
 Groth16Prover::~Groth16Prover()
 {
-    if (pinned_witness_) (void)hipHostFree(pinned_witness_);
+    if (stream2_) {
+        (void)hipStreamSynchronize(stream2_);
+        (void)hipStreamDestroy(stream2_);
+    }
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    if (ev_side_done_) (void)hipEventDestroy(ev_side_done_);
+    if (pinned_witness_) (void)hipHostFree(pinned_witness_);
 }
 
 uint64_t Groth16Prover::device_bytes() const
@@ -166,6 +175,19 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     hipStream_t s = stream_;
     StageTimer *t = timer_.enabled() ? &timer_ : nullptr;
 
+    // Witness MSMs (ar, bs1, zk on G1 and bs2 on G2) depend only on z: they
+    // run on the side stream while the main stream computes H.
+    const uint32_t cw = wplan_.c;
+    NKR_HIP(hipEventRecord(ev_start_, s));
+    NKR_HIP(hipStreamWaitEvent(stream2_, ev_start_, 0));
+    wsched_.build(z, wplan_, stream2_, t);
+    batch_w_.run({MsmJob<fq_t>{&a_, &wsched_, planes1_.get() + 0 * cw},
+                  MsmJob<fq_t>{&b1_, &wsched_, planes1_.get() + 1 * cw},
+                  MsmJob<fq_t>{&l_, &wsched_, planes1_.get() + 2 * cw}},
+                 stream2_, t);
+    batch_g2_.run({MsmJob<fq2_t>{&b2_, &wsched_, planes2_.get()}}, stream2_, t);
+    NKR_HIP(hipEventRecord(ev_side_done_, stream2_));
+
     if (t) t->begin(0, s);
     spmv3(views_[0], views_[1], views_[2], z, pa_.get(), pb_.get(), pc_.get(), s);
     if (t) t->end(0, s);
@@ -195,22 +217,17 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
         NKR_HIP(hipMemcpyAsync(h, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
     }
 
-    // MSMs: one digit sort of the witness serves ar, bs1, zk and bs2.
-    wsched_.build(z, wplan_, s, t);
-    const uint32_t cw = wplan_.c;
-    run1_.run(a_, wsched_, planes1_.get() + 0 * cw, s, t);
-    run1_.run(b1_, wsched_, planes1_.get() + 1 * cw, s, t);
-    run1_.run(l_, wsched_, planes1_.get() + 2 * cw, s, t);
-    run2_.run(b2_, wsched_, planes2_.get(), s, t);
+    // The H MSM on the main stream, then join the side stream.
     hsched_.build(pa_.get(), hplan_, s, t);
-    run1_.run(h_, hsched_, planes1_.get() + 3 * cw, s, t);
+    batch_h_.run({MsmJob<fq_t>{&h_, &hsched_, planes1_.get() + 3 * cw}}, s, t);
+    NKR_HIP(hipStreamWaitEvent(s, ev_side_done_, 0));
 
-    const auto host0 = std::chrono::steady_clock::now();
     std::vector<Xyzz<fq_t>> p1(3 * cw + hplan_.c);
     std::vector<Xyzz<fq2_t>> p2(cw);
     NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
     NKR_HIP(hipMemcpyAsync(p2.data(), planes2_.get(), p2.size() * sizeof(p2[0]), hipMemcpyDeviceToHost, s));
     NKR_HIP(hipStreamSynchronize(s));
+    const auto host0 = std::chrono::steady_clock::now();
 
     uint8_t *out = static_cast<uint8_t *>(raw_out);
     const g1_jac ar = msm_finish(p1.data() + 0 * cw, cw);
@@ -229,8 +246,11 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
         stage_ms_[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
         counters_[0] += 1;
         counters_[1] += 5;
-        counters_[2] += wsched_.entries();
-        counters_[3] += hsched_.entries();
+        uint32_t tw = 0, th = 0;
+        NKR_HIP(hipMemcpy(&tw, wsched_.totals(), sizeof(tw), hipMemcpyDeviceToHost));
+        NKR_HIP(hipMemcpy(&th, hsched_.totals(), sizeof(th), hipMemcpyDeviceToHost));
+        counters_[2] += tw;
+        counters_[3] += th;
     }
 }
 
@@ -240,7 +260,7 @@ template <class F>
 Xyzz<F> jmul(const Jacobian<F> &p, const fr_t &k_mont)
 {
     const fr_t k = fp_from_mont(k_mont);
-    return xyzz_mul_u256(jacobian_to_xyzz(p), k.l);
+    return xyzz_mul_u256_w4(jacobian_to_xyzz(p), k.l);
 }
 
 template <class F>
@@ -277,8 +297,8 @@ void groth16_blind(const nkr_g16_pk &pk, const void *raw, const fr_t &r, const f
     // C = qz + zk + s*A + r*B1 - (r*s)*delta1
     const fr_t sk = fp_from_mont(s), rk = fp_from_mont(r);
     g1_xyzz cpt = xyzz_add(qz, zk);
-    cpt = xyzz_add(cpt, xyzz_mul_u256(a, sk.l));
-    cpt = xyzz_add(cpt, xyzz_mul_u256(b1, rk.l));
+    cpt = xyzz_add(cpt, xyzz_mul_u256_w4(a, sk.l));
+    cpt = xyzz_add(cpt, xyzz_mul_u256_w4(b1, rk.l));
     cpt = xyzz_add(cpt, xyzz_neg(jmul(delta1, fp_mul(r, s))));
 
     const g1_jac A = xyzz_to_jacobian_affine(a);
