@@ -113,6 +113,10 @@ int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out);
 int nkr_g16_profile(nkr_g16_ctx *ctx, int enable);
 int nkr_g16_profile_read(const nkr_g16_ctx *ctx, double *stage_ms, uint64_t *counters);
 
+/* check_witness (protocols/groth16/common/src/protocol.cpp:69-83): *ok = 1
+ * when (A z) * (B z) = C z row by row.  Host arithmetic. */
+int nkr_g16_check_witness(const nkr_g16_circuit *circuit, const void *witness, int *ok);
+
 /* blind_proof with caller-supplied r, s (Fr, Montgomery).  Host arithmetic. */
 int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
 

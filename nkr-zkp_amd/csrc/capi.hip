@@ -161,6 +161,33 @@ int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void 
     });
 }
 
+int nkr_g16_check_witness(const nkr_g16_circuit *c, const void *witness, int *ok)
+{
+    return guarded([&] {
+        NKR_REQUIRE(c && witness && ok, "null argument");
+        const fr_t *z = static_cast<const fr_t *>(witness);
+        const uint32_t *rps[3] = {c->a_row_ptr, c->b_row_ptr, c->c_row_ptr};
+        const uint32_t *cis[3] = {c->a_col_idx, c->b_col_idx, c->c_col_idx};
+        const fr_t *vs[3] = {static_cast<const fr_t *>(c->a_values), static_cast<const fr_t *>(c->b_values),
+                             static_cast<const fr_t *>(c->c_values)};
+        const uint64_t nnz[3] = {c->a_nnz, c->b_nnz, c->c_nnz};
+        for (int m = 0; m < 3; ++m) {
+            NKR_REQUIRE(rps[m] && rps[m][0] == 0 && rps[m][c->rows] == nnz[m], "invalid CSR offsets");
+            for (uint64_t j = 0; j < nnz[m]; ++j) NKR_REQUIRE(cis[m][j] < c->cols, "out-of-range column");
+        }
+        *ok = 1;
+        for (uint64_t row = 0; row < c->rows && *ok; ++row) {
+            fr_t v[3];
+            for (int m = 0; m < 3; ++m) {
+                v[m] = fr_t::zero();
+                for (uint32_t j = rps[m][row]; j < rps[m][row + 1]; ++j)
+                    v[m] = fp_add(v[m], fp_mul(vs[m][j], z[cis[m][j]]));
+            }
+            if (fp_mul(v[0], v[1]) != v[2]) *ok = 0;
+        }
+    });
+}
+
 int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out)
 {
     return guarded([&] {

@@ -60,6 +60,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_prove_ex": [vp, vp, vp, vp, vp, vp, vp],
             "nkr_g16_blind": [ctypes.POINTER(Pk), vp, vp, vp, vp],
             "nkr_g16_prove_dev": [vp, vp, vp],
+            "nkr_g16_check_witness": [ctypes.POINTER(Circuit), vp, ctypes.POINTER(ctypes.c_int)],
             "nkr_g16_profile": [vp, i32],
             "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
             "nkr_dev_copy": [vp, vp, u64, vp],
