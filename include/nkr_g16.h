@@ -100,6 +100,16 @@ int nkr_g16_prove_ex(nkr_g16_ctx *ctx, const void *witness, void *raw_out, void 
 /* As nkr_g16_prove with the witness already in device memory (cols Fr). */
 int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out);
 
+/* prove followed by blind_proof with caller-supplied r, s (the reference's
+ * groth16.prove.total scope, protocols/groth16/app/prove.cpp:79-88) in one
+ * call: the r,s-only blinding terms are computed on the host while the GPU
+ * works.  raw_out (576 B) and proof_out (384 B) are both written.  The
+ * proving key given to nkr_g16_ctx_create must include alpha/beta/delta. */
+int nkr_g16_prove_blind(nkr_g16_ctx *ctx, const void *witness, const void *r, const void *s, void *raw_out,
+                        void *proof_out);
+int nkr_g16_prove_blind_dev(nkr_g16_ctx *ctx, const void *witness_dev, const void *r, const void *s,
+                            void *raw_out, void *proof_out);
+
 /* Stage timing with HIP events on the context's stream.  While enabled,
  * every prove adds its per-stage device time (ms) to running totals:
  *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit
@@ -120,6 +130,43 @@ int nkr_g16_check_witness(const nkr_g16_circuit *circuit, const void *witness, i
 /* blind_proof with caller-supplied r, s (Fr, Montgomery).  Host arithmetic. */
 int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
 
+/* VerificationKey (protocols/groth16/common/include/groth16/types.hpp:40-45):
+ * alpha_g1_beta_g2 is a GT element (Fq12, 384 B: c0.c0.c0 .. c1.c2.c1, each
+ * 32 B Montgomery Fq), gamma_g2 and delta_g2 are G2 (192 B), ic holds
+ * ic_len G1 points (96 B each). */
+typedef struct nkr_g16_vk {
+    const void *alpha_g1_beta_g2;
+    const void *gamma_g2;
+    const void *delta_g2;
+    const void *ic;
+    uint64_t ic_len;
+} nkr_g16_vk;
+
+/* verify (protocols/groth16/common/src/protocol.cpp:96-108): *accepted = 1
+ * when e(A, B) == alpha_g1_beta_g2 * e(IC, gamma) * e(C, delta) with
+ * IC = ic[0] + sum_i x_i ic[i+1]; 0 when it fails, when ic_len != n_public + 1
+ * (as the reference) or when a proof point is off its curve or B is outside
+ * the order-r subgroup.  public_input: n_public Fr (Montgomery).  proof:
+ * 384 B Proof.  Host arithmetic. */
+int nkr_g16_verify(const nkr_g16_vk *vk, const void *public_input, uint64_t n_public, const void *proof,
+                   int *accepted);
+
+/* alt_bn128_reduced_pairing (libff alt_bn128_pairing.cpp:542-547): gt_out =
+ * e(P, Q) for a G1 point (96 B) and a G2 point (192 B); the value of libff's
+ * final exponentiation, f^(m (q^12 - 1)/r) with m = 2x(6x^2 + 3x + 1).
+ * Host arithmetic. */
+int nkr_bn254_pairing(const void *g1, const void *g2, void *gt_out);
+
+/* prod_i e(P_i, Q_i) with one final exponentiation (k pairs, k may be 0). */
+int nkr_bn254_pairing_product(const void *g1s, const void *g2s, uint64_t k, void *gt_out);
+
+/* alt_bn128_ate_miller_loop (alt_bn128_pairing.cpp:368-420) and
+ * alt_bn128_final_exponentiation (:226-237) separately.  The Miller value is
+ * this library's (affine lines); only its final exponentiation is defined to
+ * agree with the reference. */
+int nkr_bn254_miller_loop(const void *g1, const void *g2, void *f_out);
+int nkr_bn254_final_exponentiation(const void *f, void *gt_out);
+
 /* Quotient stage alone: H from A*z, B*z, C*z (rows Fr each, host pointers). */
 int nkr_g16_quotient(const void *az, const void *bz, const void *cz, uint64_t rows, void *h, int device);
 
@@ -127,6 +174,12 @@ int nkr_g16_quotient(const void *az, const void *bz, const void *cz, uint64_t ro
 /* kind: 0 FFT, 1 iFFT, 2 cosetFFT (g = 5), 3 icosetFFT.  n = 2^log_n elements. */
 int nkr_fr_ntt(void *data, uint32_t log_n, int kind, int device);               /* host memory */
 int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream);    /* device memory */
+
+/* NTT workgroup shape for contexts and domains created afterwards:
+ * 2^lds_log elements per workgroup (10 or 11), 2^radix_log elements per
+ * thread per register round (2 or 3).  A tuning knob; the default is chosen
+ * by measurement. */
+int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log);
 
 /* y = M * z for one CSR matrix (host memory). */
 int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,

@@ -6,10 +6,12 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/nkr_g16.h"
 #include "msm.h"
 #include "ntt.h"
+#include "pairing.h"
 #include "prover.h"
 #include "setup.h"
 #include "spmv.h"
@@ -196,6 +198,30 @@ int nkr_g16_prove_dev(nkr_g16_ctx *ctx, const void *witness_dev, void *raw_out)
     });
 }
 
+static int prove_blind(nkr_g16_ctx *ctx, const void *witness, bool dev, const void *r, const void *s,
+                       void *raw_out, void *proof_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx && r && s, "null argument");
+        fr_t rr, ss;
+        std::memcpy(&rr, r, sizeof(rr));
+        std::memcpy(&ss, s, sizeof(ss));
+        ctx->prover->prove_blind(witness, dev, rr, ss, raw_out, proof_out);
+    });
+}
+
+int nkr_g16_prove_blind(nkr_g16_ctx *ctx, const void *witness, const void *r, const void *s, void *raw_out,
+                        void *proof_out)
+{
+    return prove_blind(ctx, witness, false, r, s, raw_out, proof_out);
+}
+
+int nkr_g16_prove_blind_dev(nkr_g16_ctx *ctx, const void *witness_dev, const void *r, const void *s,
+                            void *raw_out, void *proof_out)
+{
+    return prove_blind(ctx, witness_dev, true, r, s, raw_out, proof_out);
+}
+
 int nkr_g16_profile(nkr_g16_ctx *ctx, int enable)
 {
     return guarded([&] {
@@ -220,6 +246,81 @@ int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const vo
         std::memcpy(&rr, r, sizeof(rr));
         std::memcpy(&ss, s, sizeof(ss));
         groth16_blind(*pk, raw, rr, ss, proof_out);
+    });
+}
+
+int nkr_g16_verify(const nkr_g16_vk *vk, const void *public_input, uint64_t n_public, const void *proof,
+                   int *accepted)
+{
+    return guarded([&] {
+        NKR_REQUIRE(vk && proof && accepted, "null argument");
+        NKR_REQUIRE(vk->alpha_g1_beta_g2 && vk->gamma_g2 && vk->delta_g2, "verification key has null members");
+        NKR_REQUIRE(vk->ic || vk->ic_len == 0, "verification key has a null ic array");
+        NKR_REQUIRE(public_input || n_public == 0, "null public input");
+        *accepted = 0;
+        // copies, so the host arithmetic never reads through unaligned pointers
+        fq12_t ab;
+        g2_jac gamma, delta;
+        std::memcpy(&ab, vk->alpha_g1_beta_g2, sizeof(ab));
+        std::memcpy(&gamma, vk->gamma_g2, sizeof(gamma));
+        std::memcpy(&delta, vk->delta_g2, sizeof(delta));
+        std::vector<g1_jac> ic(vk->ic_len);
+        if (vk->ic_len) std::memcpy(ic.data(), vk->ic, vk->ic_len * sizeof(g1_jac));
+        std::vector<fr_t> x(n_public);
+        if (n_public) std::memcpy(x.data(), public_input, n_public * sizeof(fr_t));
+        struct {
+            g1_jac a;
+            g2_jac b;
+            g1_jac c;
+        } pr;
+        static_assert(sizeof(pr) == 384, "Proof layout");
+        std::memcpy(&pr, proof, sizeof(pr));
+        const VerifyingKeyView view{&ab, &gamma, &delta, ic.data(), ic.size()};
+        *accepted = groth16_verify(view, x.data(), n_public, pr.a, pr.b, pr.c) ? 1 : 0;
+    });
+}
+
+int nkr_bn254_pairing(const void *g1, const void *g2, void *gt_out)
+{
+    return nkr_bn254_pairing_product(g1, g2, 1, gt_out);
+}
+
+int nkr_bn254_pairing_product(const void *g1s, const void *g2s, uint64_t k, void *gt_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(gt_out && ((g1s && g2s) || k == 0), "null argument");
+        std::vector<g1_jac> p(k);
+        std::vector<g2_jac> q(k);
+        if (k) {
+            std::memcpy(p.data(), g1s, k * sizeof(g1_jac));
+            std::memcpy(q.data(), g2s, k * sizeof(g2_jac));
+        }
+        const fq12_t r = bn254_pairing_product(p.data(), q.data(), k);
+        std::memcpy(gt_out, &r, sizeof(r));
+    });
+}
+
+int nkr_bn254_miller_loop(const void *g1, const void *g2, void *f_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(g1 && g2 && f_out, "null argument");
+        g1_jac p;
+        g2_jac q;
+        std::memcpy(&p, g1, sizeof(p));
+        std::memcpy(&q, g2, sizeof(q));
+        const fq12_t f = bn254_miller_loop(p, q);
+        std::memcpy(f_out, &f, sizeof(f));
+    });
+}
+
+int nkr_bn254_final_exponentiation(const void *f, void *gt_out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(f && gt_out, "null argument");
+        fq12_t x;
+        std::memcpy(&x, f, sizeof(x));
+        const fq12_t r = bn254_final_exponentiation(x);
+        std::memcpy(gt_out, &r, sizeof(r));
     });
 }
 
@@ -279,6 +380,24 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream)
         if (!scratch) scratch = new DeviceBuffer<fr_t>();
         scratch->ensure(size_t(1) << log_n);
         dom->standalone(static_cast<fr_t *>(data_dev), scratch->get(), kind, s);
+    });
+}
+
+int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log)
+{
+    return guarded([&] {
+        ntt_set_shape(lds_log, radix_log);
+        // Cached standalone domains were built with the old shape.
+        std::lock_guard<std::mutex> lock(g_ntt_mutex);
+        bool any = false;
+        for (auto &row : g_ntt_cache)
+            for (NttDomain *d : row) any = any || d;
+        if (any) NKR_HIP(hipDeviceSynchronize());
+        for (auto &row : g_ntt_cache)
+            for (NttDomain *&d : row) {
+                delete d;
+                d = nullptr;
+            }
     });
 }
 

@@ -60,6 +60,7 @@ private:
              hipStream_t stream) const;
 
     unsigned log_n_;
+    unsigned lds_log_ = 11, radix_log_ = 3;
     unsigned m_lo_;        // bits of the low twiddle table index
     unsigned log_roots_;   // sub-transform roots are of order 2^log_roots_
     std::vector<Pass> passes_;
@@ -69,6 +70,10 @@ private:
     DeviceBuffer<fr_t> tw_lo_[2], tw_hi_[2], roots_[2];
     DeviceBuffer<fr_t> coset_nat_, coset_rev_, icoset_rev_, consts_;
 };
+
+// Workgroup shape for domains built afterwards: 2^lds_log LDS elements per
+// workgroup (10 or 11) and 2^radix_log elements per thread per round (2 or 3).
+void ntt_set_shape(unsigned lds_log, unsigned radix_log);
 
 // Fr constants shared by host code.
 fr_t fr_from_u64(uint64_t x);

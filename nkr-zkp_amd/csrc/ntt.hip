@@ -10,7 +10,13 @@ namespace nkr {
 
 namespace {
 
-constexpr unsigned kThreads = 256;
+constexpr unsigned kMaxThreads = 512;
+
+// Workgroup shape of the pass kernels: 2^lds_log elements of LDS per
+// workgroup and 2^radix_log elements per thread per register round
+// (threads = 2^(lds_log - radix_log)).  Chosen by measurement on MI355X.
+unsigned g_lds_log = 11;
+unsigned g_radix_log = 3;
 
 struct PassArgs {
     fr_t *data;
@@ -84,8 +90,8 @@ __device__ __forceinline__ void ntt_round(fr_t *smem, const PassArgs &a, uint32_
     }
 }
 
-template <bool DIT>
-__global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs a)
+template <bool DIT, int RMAX>
+__global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
 {
     extern __shared__ fr_t smem[];
     const uint32_t G = 1u << a.np;
@@ -118,13 +124,13 @@ __global__ __launch_bounds__(kThreads) void k_ntt_pass(PassArgs a)
     }
     __syncthreads();
 
-    // Sub-transform in rounds of up to 3 radix-2 stages: each thread keeps a
-    // group of 8 elements in registers for the whole round, so LDS is
-    // visited once per 3 stages.  DIF rounds run stages t..t+r-1, DIT rounds
-    // the same stages in reverse (its transpose).
+    // Sub-transform in rounds of up to RMAX radix-2 stages: each thread keeps
+    // a group of 2^RMAX elements in registers for the whole round, so LDS is
+    // visited once per RMAX stages.  DIF rounds run stages t..t+r-1, DIT
+    // rounds the same stages in reverse (its transpose).
     const uint32_t root_shift = a.log_roots - a.np;
     for (uint32_t done = 0; done < a.np;) {
-        const uint32_t r = min(3u, a.np - done);
+        const uint32_t r = min((uint32_t)RMAX, a.np - done);
         const uint32_t t = DIT ? (a.np - done - r) : done;
         if (r == 3)
             ntt_round<DIT, 3>(smem, a, t, cols, root_shift);
@@ -210,6 +216,14 @@ constexpr unsigned kTwoAdicity = 28;
 
 }  // namespace
 
+void ntt_set_shape(unsigned lds_log, unsigned radix_log)
+{
+    NKR_REQUIRE(lds_log == 10 || lds_log == 11, "LDS elements per workgroup must be 2^10 or 2^11");
+    NKR_REQUIRE(radix_log == 2 || radix_log == 3, "register radix must be 2^2 or 2^3");
+    g_lds_log = lds_log;
+    g_radix_log = radix_log;
+}
+
 fr_t fr_from_u64(uint64_t x)
 {
     fr_t t = fr_t::zero();
@@ -249,14 +263,17 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
     const fr_t g_inv = fp_inv(g);
     zinv_ = fp_inv(fp_sub(fr_pow_u64(g, n), fr_t::one()));
 
-    // pass plan: sub-transforms of at most 11 bits (2048 elements)
+    // pass plan: sub-transforms of at most lds_log bits
+    lds_log_ = g_lds_log;
+    radix_log_ = g_radix_log;
     std::vector<unsigned> split;
-    if (log_n <= 11) {
+    if (log_n <= lds_log_) {
         split = {log_n};
-    } else if (log_n <= 22) {
+    } else if (log_n <= 2 * lds_log_) {
         split = {(log_n + 1) / 2, log_n / 2};
     } else {
         const unsigned a = (log_n + 2) / 3, b = (log_n + 1) / 3, c = log_n / 3;
+        NKR_REQUIRE(a <= lds_log_, "NTT size too large for the configured workgroup shape");
         split = {a, b, c};
     }
     unsigned S = 0;
@@ -266,7 +283,7 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
         p.np = np;
         p.log_L = log_n - S - np;
         const unsigned groups_log = log_n - np;
-        unsigned lc = std::min(groups_log, 11u - np);
+        unsigned lc = std::min(groups_log, lds_log_ - np);
         if (p.log_L > 0 && p.log_L < lc) lc = p.log_L;
         p.log_cols = lc;
         passes_.push_back(p);
@@ -328,11 +345,20 @@ void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const f
         a.m_lo = m_lo_;
         a.twiddle = (pi != P - 1) ? 1u : 0u;
         const uint32_t blocks = (uint32_t)((size_t(1) << (log_n_ - p.np)) >> p.log_cols);
-        const size_t lds = (size_t(sizeof(fr_t)) << p.np) << p.log_cols;
-        if (dit)
-            k_ntt_pass<true><<<blocks, kThreads, lds, stream>>>(a);
-        else
-            k_ntt_pass<false><<<blocks, kThreads, lds, stream>>>(a);
+        const uint32_t elems = (1u << p.np) << p.log_cols;
+        const size_t lds = size_t(sizeof(fr_t)) * elems;
+        const uint32_t threads = std::min(kMaxThreads, std::max(64u, elems >> radix_log_));
+        if (radix_log_ >= 3) {
+            if (dit)
+                k_ntt_pass<true, 3><<<blocks, threads, lds, stream>>>(a);
+            else
+                k_ntt_pass<false, 3><<<blocks, threads, lds, stream>>>(a);
+        } else {
+            if (dit)
+                k_ntt_pass<true, 2><<<blocks, threads, lds, stream>>>(a);
+            else
+                k_ntt_pass<false, 2><<<blocks, threads, lds, stream>>>(a);
+        }
         NKR_LAUNCH_CHECK();
     }
 }

@@ -31,6 +31,11 @@ public:
     void prove(const void *witness, void *raw_out, void *az, void *bz, void *cz, void *h);
     // witness already in device memory
     void prove_dev(const fr_t *witness_dev, void *raw_out);
+    // prove followed by blind_proof with the given r, s; the parts of the
+    // blinding that depend only on r and s run on the host while the GPU
+    // works.  witness is host memory, or device memory when on_device.
+    void prove_blind(const void *witness, bool on_device, const fr_t &r, const fr_t &s, void *raw_out,
+                     void *proof_out);
 
     static constexpr int kStages = 7;
     void set_profiling(bool on);
@@ -61,7 +66,19 @@ private:
     DeviceBuffer<Xyzz<fq2_t>> planes2_;
     void *pinned_witness_ = nullptr;
 
-    void run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h);
+    struct BlindJob {
+        fr_t r, s;
+        void *proof_out;
+    };
+    void run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind = nullptr);
+
+    // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)
+    // and the key's group elements, for blinding.
+    bool have_blind_ = false;
+    std::vector<Affine<fq_t>> tab_alpha1_, tab_beta1_, tab_delta1_;
+    std::vector<Affine<fq2_t>> tab_delta2_;
+    g1_jac alpha1_, beta1_;
+    g2_jac beta2_;
     StageTimer timer_;
     double stage_ms_[kStages] = {};
     uint64_t counters_[4] = {};

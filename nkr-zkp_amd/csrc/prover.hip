@@ -3,6 +3,8 @@
 #include <chrono>
 #include <cstring>
 
+#include "setup.h"
+
 namespace nkr {
 
 namespace {
@@ -107,6 +109,21 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
         NKR_HIP(hipStreamSynchronize(stream_));
     }
 
+    if (pk.alpha_g1 && pk.beta_g1 && pk.beta_g2 && pk.delta_g1 && pk.delta_g2) {
+        std::memcpy(&alpha1_, pk.alpha_g1, sizeof(alpha1_));
+        std::memcpy(&beta1_, pk.beta_g1, sizeof(beta1_));
+        std::memcpy(&beta2_, pk.beta_g2, sizeof(beta2_));
+        g1_jac d1;
+        g2_jac d2;
+        std::memcpy(&d1, pk.delta_g1, sizeof(d1));
+        std::memcpy(&d2, pk.delta_g2, sizeof(d2));
+        tab_alpha1_ = FixedBaseMul<fq_t>(alpha1_, stream_).table_host(stream_);
+        tab_beta1_ = FixedBaseMul<fq_t>(beta1_, stream_).table_host(stream_);
+        tab_delta1_ = FixedBaseMul<fq_t>(d1, stream_).table_host(stream_);
+        tab_delta2_ = FixedBaseMul<fq2_t>(d2, stream_).table_host(stream_);
+        have_blind_ = true;
+    }
+
     z_.allocate(cols_);
     pa_.allocate(rows_);
     pb_.allocate(rows_);
@@ -170,7 +187,53 @@ void Groth16Prover::profile_read(double *stage_ms, uint64_t *counters) const
     for (int i = 0; i < 4; ++i) counters[i] = counters_[i];
 }
 
-void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h)
+void Groth16Prover::prove_blind(const void *witness, bool on_device, const fr_t &r, const fr_t &s, void *raw_out,
+                                void *proof_out)
+{
+    NKR_REQUIRE(witness && raw_out && proof_out, "null witness or output");
+    NKR_REQUIRE(have_blind_, "the proving key given to prepare lacks alpha/beta/delta");
+    NKR_HIP(hipSetDevice(device_));
+    const fr_t *z = static_cast<const fr_t *>(witness);
+    if (!on_device) {
+        const size_t fr_bytes = (size_t)cols_ * sizeof(fr_t);
+        std::memcpy(pinned_witness_, witness, fr_bytes);
+        NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, stream_));
+        z = z_.get();
+    }
+    const BlindJob job{r, s, proof_out};
+    run(z, raw_out, nullptr, nullptr, nullptr, nullptr, &job);
+}
+
+namespace {
+
+// a*P + b*Q with shared doublings (Straus, 4-bit windows).
+template <class F>
+Xyzz<F> straus2(const Xyzz<F> &P, const fr_t &a_mont, const Xyzz<F> &Q, const fr_t &b_mont)
+{
+    const fr_t a = fp_from_mont(a_mont), b = fp_from_mont(b_mont);
+    Xyzz<F> tp[16], tq[16];
+    tp[0] = tq[0] = Xyzz<F>::infinity();
+    tp[1] = P;
+    tq[1] = Q;
+    for (int i = 2; i < 16; ++i) {
+        tp[i] = xyzz_add(tp[i - 1], P);
+        tq[i] = xyzz_add(tq[i - 1], Q);
+    }
+    Xyzz<F> acc = Xyzz<F>::infinity();
+    for (int nib = 63; nib >= 0; --nib) {
+        if (!acc.is_inf())
+            for (int d = 0; d < 4; ++d) acc = xyzz_dbl(acc);
+        const uint32_t sh = (nib & 7) * 4;
+        const uint32_t va = (a.l[nib >> 3] >> sh) & 15u, vb = (b.l[nib >> 3] >> sh) & 15u;
+        if (va) acc = xyzz_add(acc, tp[va]);
+        if (vb) acc = xyzz_add(acc, tq[vb]);
+    }
+    return acc;
+}
+
+}  // namespace
+
+void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind)
 {
     hipStream_t s = stream_;
     StageTimer *t = timer_.enabled() ? &timer_ : nullptr;
@@ -226,6 +289,19 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     std::vector<Xyzz<fq2_t>> p2(cw);
     NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
     NKR_HIP(hipMemcpyAsync(p2.data(), planes2_.get(), p2.size() * sizeof(p2[0]), hipMemcpyDeviceToHost, s));
+
+    // Blinding terms that need only r and s, while the GPU works:
+    // A0 = alpha_1 + r delta_1, B0 = beta_2 + s delta_2, and
+    // C0 = s alpha_1 + r beta_1 + (r s) delta_1, using
+    // s*A + r*B_1 - (r s) delta_1 = s alpha_1 + s ar + r beta_1 + r bs1 + (r s) delta_1.
+    g1_xyzz A0, C0;
+    g2_xyzz B0;
+    if (blind) {
+        A0 = xyzz_add(jacobian_to_xyzz(alpha1_), fixed_base_mul_host(tab_delta1_, blind->r));
+        B0 = xyzz_add(jacobian_to_xyzz(beta2_), fixed_base_mul_host(tab_delta2_, blind->s));
+        C0 = xyzz_add(fixed_base_mul_host(tab_alpha1_, blind->s), fixed_base_mul_host(tab_beta1_, blind->r));
+        C0 = xyzz_add(C0, fixed_base_mul_host(tab_delta1_, fp_mul(blind->r, blind->s)));
+    }
     NKR_HIP(hipStreamSynchronize(s));
     const auto host0 = std::chrono::steady_clock::now();
 
@@ -240,6 +316,20 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     std::memcpy(out + 192, &zk, 96);
     std::memcpy(out + 288, &qz, 96);
     std::memcpy(out + 384, &bs2, 192);
+
+    if (blind) {
+        const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
+        const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(A0, xar));
+        const g2_jac B = xyzz_to_jacobian_affine(xyzz_add(B0, jacobian_to_xyzz(bs2)));
+        g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
+        C = xyzz_add(C, C0);
+        C = xyzz_add(C, straus2(xar, blind->s, xbs1, blind->r));
+        const g1_jac Cj = xyzz_to_jacobian_affine(C);
+        uint8_t *o = static_cast<uint8_t *>(blind->proof_out);
+        std::memcpy(o, &A, 96);
+        std::memcpy(o + 96, &B, 192);
+        std::memcpy(o + 288, &Cj, 96);
+    }
 
     if (t) {
         t->collect(stage_ms_);

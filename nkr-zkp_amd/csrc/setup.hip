@@ -68,6 +68,34 @@ void FixedBaseMul<F>::run(const fr_t *scalars_dev, uint64_t n, Jacobian<F> *out_
     NKR_LAUNCH_CHECK();
 }
 
+template <class F>
+std::vector<Affine<F>> FixedBaseMul<F>::table_host(hipStream_t stream) const
+{
+    std::vector<Affine<F>> t(32 * 256);
+    NKR_HIP(hipMemcpyAsync(t.data(), table_.get(), t.size() * sizeof(Affine<F>), hipMemcpyDeviceToHost, stream));
+    NKR_HIP(hipStreamSynchronize(stream));
+    return t;
+}
+
+// Host: k * base from a comb table (32 mixed additions, no doublings).
+template <class F>
+Xyzz<F> fixed_base_mul_host(const std::vector<Affine<F>> &table, const fr_t &k_mont)
+{
+    const fr_t k = fp_from_mont(k_mont);
+    Xyzz<F> acc = Xyzz<F>::infinity();
+    for (uint32_t j = 0; j < 32; ++j) {
+        const uint32_t d = (k.l[j >> 2] >> ((j & 3u) * 8)) & 255u;
+        if (d == 0) continue;
+        const Affine<F> &t = table[j * 256 + d];
+        if (affine_is_inf(t)) continue;
+        acc = xyzz_add_affine(acc, t.x, t.y);
+    }
+    return acc;
+}
+
+template Xyzz<fq_t> fixed_base_mul_host(const std::vector<Affine<fq_t>> &, const fr_t &);
+template Xyzz<fq2_t> fixed_base_mul_host(const std::vector<Affine<fq2_t>> &, const fr_t &);
+
 template class FixedBaseMul<fq_t>;
 template class FixedBaseMul<fq2_t>;
 

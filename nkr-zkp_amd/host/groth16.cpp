@@ -118,6 +118,14 @@ Proof blind_proof(const ProvingKey &key, const RawProof &raw, const Scalar &r, c
     return p;
 }
 
+bool verify(const VerificationKey &key, const std::vector<Scalar> &public_input, const Proof &proof)
+{
+    const nkr_g16_vk vk{&key.alpha_g1_beta_g2, &key.gamma_g2, &key.delta_g2, key.ic.data(), key.ic.size()};
+    int accepted = 0;
+    check(nkr_g16_verify(&vk, public_input.data(), public_input.size(), &proof, &accepted), "verify");
+    return accepted != 0;
+}
+
 std::array<Scalar, 2> sample_blinding(uint64_t seed)
 {
     // Fp_model::random_element (libff fp.tcc:719-744): four words, bits above

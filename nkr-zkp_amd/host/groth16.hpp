@@ -100,6 +100,8 @@ std::vector<std::string> available_backends();
 Proof blind_proof(const ProvingKey &key, const RawProof &raw, const Scalar &r, const Scalar &s);
 // Seeded r, s by libff's random_element rule on mt19937_64(seed).
 std::array<Scalar, 2> sample_blinding(uint64_t seed);
+// verify (protocol.cpp:96-108): the pairing check on the host.
+bool verify(const VerificationKey &key, const std::vector<Scalar> &public_input, const Proof &proof);
 
 // ------------------------------------------------------------- validation
 bool validate_circuit(const Circuit &circuit, std::string &error);

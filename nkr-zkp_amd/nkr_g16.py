@@ -39,6 +39,11 @@ class Pk(ctypes.Structure):
         "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2")]
 
 
+class Vk(ctypes.Structure):
+    _fields_ = [("alpha_g1_beta_g2", ctypes.c_void_p), ("gamma_g2", ctypes.c_void_p),
+                ("delta_g2", ctypes.c_void_p), ("ic", ctypes.c_void_p), ("ic_len", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -61,9 +66,12 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_blind": [ctypes.POINTER(Pk), vp, vp, vp, vp],
             "nkr_g16_prove_dev": [vp, vp, vp],
             "nkr_g16_check_witness": [ctypes.POINTER(Circuit), vp, ctypes.POINTER(ctypes.c_int)],
+            "nkr_g16_prove_blind": [vp, vp, vp, vp, vp, vp],
+            "nkr_g16_prove_blind_dev": [vp, vp, vp, vp, vp, vp],
             "nkr_g16_profile": [vp, i32],
             "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
             "nkr_dev_copy": [vp, vp, u64, vp],
+            "nkr_ntt_set_shape": [u32, u32],
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
@@ -86,6 +94,11 @@ def lib() -> ctypes.CDLL:
             "nkr_memcpy_htod": [vp, vp, u64],
             "nkr_memcpy_dtoh": [vp, vp, u64],
             "nkr_device_synchronize": [],
+            "nkr_g16_verify": [ctypes.POINTER(Vk), vp, u64, vp, ctypes.POINTER(ctypes.c_int)],
+            "nkr_bn254_pairing": [vp, vp, vp],
+            "nkr_bn254_pairing_product": [vp, vp, u64, vp],
+            "nkr_bn254_miller_loop": [vp, vp, vp],
+            "nkr_bn254_final_exponentiation": [vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -281,6 +294,17 @@ class Prover:
         _check(lib().nkr_g16_prove_ex(self.h, _p(w), _p(raw), *[_p(b) for b in bufs]))
         return raw, tuple(bufs)
 
+    def prove_blind(self, witness, r, s):
+        """prove + blind_proof in one call; returns (raw words, proof words)."""
+        w = _words(witness, 4)
+        assert w.shape[0] == self.cols
+        r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+        s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+        raw = np.zeros(72, dtype=np.uint64)
+        proof = np.zeros(48, dtype=np.uint64)
+        _check(lib().nkr_g16_prove_blind(self.h, _p(w), _p(r), _p(s), _p(raw), _p(proof)))
+        return raw, proof
+
     def blind(self, raw, r, s) -> np.ndarray:
         raw = np.ascontiguousarray(raw, dtype=np.uint64)
         r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
@@ -316,3 +340,55 @@ def blind(pk, raw, r, s) -> np.ndarray:
     out = np.zeros(48, dtype=np.uint64)
     _check(lib().nkr_g16_blind(ctypes.byref(p), _p(raw), _p(r), _p(s), _p(out)))
     return out
+
+
+# ------------------------------------------------------------ verifier (host)
+GT_WORDS = 48  # Fq12: 12 Fq elements of 4 words
+
+
+def pairing_product(g1s, g2s) -> np.ndarray:
+    """prod_i e(P_i, Q_i): g1s (k, 12) words, g2s (k, 24) words -> GT (48 words)."""
+    g1s = _words(g1s, 12)
+    g2s = _words(g2s, 24)
+    assert len(g1s) == len(g2s)
+    out = np.zeros(GT_WORDS, dtype=np.uint64)
+    _check(lib().nkr_bn254_pairing_product(_p(g1s), _p(g2s), len(g1s), _p(out)))
+    return out
+
+
+def pairing(g1, g2) -> np.ndarray:
+    g1 = _words(g1, 12)
+    g2 = _words(g2, 24)
+    out = np.zeros(GT_WORDS, dtype=np.uint64)
+    _check(lib().nkr_bn254_pairing(_p(g1), _p(g2), _p(out)))
+    return out
+
+
+def miller_loop(g1, g2) -> np.ndarray:
+    g1 = _words(g1, 12)
+    g2 = _words(g2, 24)
+    out = np.zeros(GT_WORDS, dtype=np.uint64)
+    _check(lib().nkr_bn254_miller_loop(_p(g1), _p(g2), _p(out)))
+    return out
+
+
+def final_exponentiation(f) -> np.ndarray:
+    f = np.ascontiguousarray(f, dtype=np.uint64).reshape(GT_WORDS)
+    out = np.zeros(GT_WORDS, dtype=np.uint64)
+    _check(lib().nkr_bn254_final_exponentiation(_p(f), _p(out)))
+    return out
+
+
+def verify(vk, public_input, proof) -> bool:
+    """Groth16 verify (protocol.cpp:96-108).  vk: object with alpha_g1_beta_g2
+    (48 words), gamma_g2, delta_g2 (24 words) and ic ((n+1, 12) words)."""
+    ab = np.ascontiguousarray(vk.alpha_g1_beta_g2, dtype=np.uint64).reshape(GT_WORDS)
+    gamma = _words(vk.gamma_g2, 24)
+    delta = _words(vk.delta_g2, 24)
+    ic = _words(vk.ic, 12)
+    x = _words(public_input, 4) if len(public_input) else np.zeros((0, 4), dtype=np.uint64)
+    proof = np.ascontiguousarray(proof, dtype=np.uint64).reshape(48)
+    v = Vk(_p(ab), _p(gamma), _p(delta), _p(ic), len(ic))
+    ok = ctypes.c_int(0)
+    _check(lib().nkr_g16_verify(ctypes.byref(v), _p(x), len(x), _p(proof), ctypes.byref(ok)))
+    return bool(ok.value)

@@ -35,6 +35,20 @@ def write_proving_key(path, pk):
     open(path, "wb").write(out)
 
 
+def write_verification_key(path, vk):
+    """alpha_g1_beta_g2 (GT), gamma_g2, delta_g2, then the ic vector (io.cpp:79-84)."""
+    out = MAGIC + struct.pack("<I", 1)
+    for k in ("alpha_g1_beta_g2", "gamma_g2", "delta_g2"):
+        out += np.ascontiguousarray(np.asarray(getattr(vk, k), dtype=np.uint64)).tobytes()
+    out += _vec(np.asarray(vk.ic, dtype=np.uint64))
+    open(path, "wb").write(out)
+
+
+def write_proof(path, proof):
+    open(path, "wb").write(MAGIC + struct.pack("<I", 1) +
+                           np.ascontiguousarray(np.asarray(proof, dtype=np.uint64)).tobytes())
+
+
 def read_proof(path) -> np.ndarray:
     data = open(path, "rb").read()
     assert data[:4] == MAGIC and struct.unpack("<I", data[4:8])[0] == 1

@@ -33,6 +33,9 @@ def test_prove_golden_matmul2():
     assert np.array_equal(pv.blind(raw, e.r, e.s), e.proof)
     # proving twice with the same context gives the same bytes
     assert np.array_equal(pv.prove(e.witness), raw)
+    # fused prove + blind (comb tables, Straus) gives the same proof
+    raw2, proof2 = pv.prove_blind(e.witness, e.r, e.s)
+    assert np.array_equal(raw2, e.raw) and np.array_equal(proof2, e.proof)
     pv.close()
 
 
@@ -87,7 +90,10 @@ def test_prove_matmul16_domain_2_13_vs_oracle():
     assert np.array_equal(raw, affine_raw(want_raw))
     rs = O.fr_random(0x4E4B52 ^ 0x72, 2)
     proof = pv.blind(raw, rs[0], rs[1])
-    assert np.array_equal(proof, affine_proof(O.blind(pk, want_raw, rs[0], rs[1])))
+    want_proof = affine_proof(O.blind(pk, want_raw, rs[0], rs[1]))
+    assert np.array_equal(proof, want_proof)
+    raw2, proof2 = pv.prove_blind(w, rs[0], rs[1])
+    assert np.array_equal(raw2, raw) and np.array_equal(proof2, want_proof)
     pv.close()
 
 
