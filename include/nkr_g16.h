@@ -130,6 +130,31 @@ int nkr_g16_check_witness(const nkr_g16_circuit *circuit, const void *witness, i
 /* blind_proof with caller-supplied r, s (Fr, Montgomery).  Host arithmetic. */
 int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
 
+/* Output buffers of nkr_g16_setup, caller-allocated host memory: the
+ * ProvingKey (types.hpp:30-38) with a_query, b_g1_query, l_query: cols G1;
+ * b_g2_query: cols G2; h_query: rows G1; alpha/beta/delta points; and the
+ * VerificationKey with ic: public_inputs + 1 G1. */
+typedef struct {
+    void *a_query, *b_g1_query, *b_g2_query, *l_query, *h_query;
+    void *alpha_g1, *beta_g1, *beta_g2, *delta_g1, *delta_g2;
+} nkr_g16_pk_out;
+
+typedef struct {
+    void *alpha_g1_beta_g2; /* GT, 384 B */
+    void *gamma_g2, *delta_g2;
+    void *ic;
+} nkr_g16_vk_out;
+
+/* Backend::setup (protocols/groth16/common/include/groth16/backend.hpp:20) as
+ * CpuBackend::setup computes it (backends/cpu/src/backend.cpp:41-154) and
+ * HipBackend::setup accelerates it (backends/hip/src/backend.hip:111-268),
+ * with the trapdoors supplied by the caller instead of drawn inside:
+ * trapdoors = 5 Fr (Montgomery) t, alpha, beta, gamma, delta.  Query
+ * scalars, column evaluations and fixed-base multiplications run on the GPU;
+ * alpha_g1_beta_g2 is the host pairing.  Points are written normalised. */
+int nkr_g16_setup(const nkr_g16_circuit *circuit, const void *trapdoors, int device, const nkr_g16_pk_out *pk,
+                  const nkr_g16_vk_out *vk);
+
 /* VerificationKey (protocols/groth16/common/include/groth16/types.hpp:40-45):
  * alpha_g1_beta_g2 is a GT element (Fq12, 384 B: c0.c0.c0 .. c1.c2.c1, each
  * 32 B Montgomery Fq), gamma_g2 and delta_g2 are G2 (192 B), ic holds

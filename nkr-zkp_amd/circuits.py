@@ -168,6 +168,215 @@ def matmul(s: int, seed: int, density: str = "full", wide: bool = False) -> R1cs
                 witness)
 
 
+class R1csBuilder:
+    """The reference's R1CSManager semantics (protocols/groth16/common/include/
+    r1cs/r1cs.hpp:26-99,104-160) on canonical integers mod r: public ids
+    count up from 1, private ids from 2^63; a linear combination is a dict
+    id -> coefficient whose terms are emitted in id order, so public columns
+    precede private ones; LinearCombination() holds {0: 0} and
+    LinearCombination(c) holds {0: c}; add_constraint(out, in) is
+    in * 1 = out; columns map to 1 + public..., then private...; rows and
+    columns are padded to powers of two."""
+
+    HALF = 1 << 63
+
+    def __init__(self):
+        self.n_public = 1
+        self.n_private = self.HALF
+        self.rows = []          # (a, b, c) dicts
+        self.values = {}        # id -> witness value
+
+    def public(self, value=0):
+        i = self.n_public
+        self.n_public += 1
+        self.values[i] = value % R
+        return i
+
+    def private(self, value=0):
+        i = self.n_private
+        self.n_private += 1
+        self.values[i] = value % R
+        return i
+
+    @staticmethod
+    def lc(*terms, const=None):
+        """terms: (id, coeff) pairs; const: None for a Variable-style LC (no
+        constant term), else the constant at id 0."""
+        d = {} if const is None else {0: const % R}
+        for i, c in terms:
+            d[i] = (d.get(i, 0) + c) % R
+        return d
+
+    @staticmethod
+    def add(a, b):
+        d = dict(a)
+        for i, c in b.items():
+            d[i] = (d.get(i, 0) + c) % R
+        return d
+
+    @staticmethod
+    def scale(a, s):
+        return {i: c * s % R for i, c in a.items()}
+
+    def constrain(self, out, a, b=None):
+        self.rows.append((a, {0: 1} if b is None else b, out))
+
+    def dense(self, i):
+        return i if i < self.HALF else i - self.HALF + self.n_public
+
+    def finish(self) -> R1cs:
+        live_rows = len(self.rows)
+        live_cols = self.n_public + (self.n_private - self.HALF)
+        rows = 1 << max(live_rows - 1, 0).bit_length()
+        cols = 1 << max(live_cols - 1, 0).bit_length()
+        mats = []
+        for k in range(3):
+            rp, ci, va = [0], [], []
+            for row in self.rows:
+                for i in sorted(row[k]):
+                    ci.append(self.dense(i))
+                    va.append(row[k][i])
+                rp.append(len(ci))
+            rp += [rp[-1]] * (rows - live_rows)
+            mats.append((np.array(rp, dtype=np.uint32), np.array(ci, dtype=np.uint32).reshape(-1),
+                         fr_mont(va) if va else np.zeros((0, 4), dtype=np.uint64)))
+        z = [0] * cols
+        z[0] = 1
+        for i, v in self.values.items():
+            z[self.dense(i)] = v
+        return R1cs(rows, cols, self.n_public - 1, mats[0], mats[1], mats[2], fr_mont(z))
+
+
+def fibonacci(steps: int, seed: int) -> R1cs:
+    """build_fibonacci (protocols/groth16/app/gen.cpp:87-120): public F0, F1 and
+    F_steps, one private term per step with F_i * 1 = F_(i-2) + F_(i-1), and a
+    final F_steps * 1 = current.  F0, F1 uniform Fr from mt19937_64(seed ^ 0xF1B)."""
+    if steps < 2:
+        raise ValueError("fibonacci steps must be at least 2")
+    rng = MT64(seed ^ 0xF1B)
+    rinv = pow(MONT, -1, R)
+    f0 = fr_random_mont(rng) * rinv % R
+    f1 = fr_random_mont(rng) * rinv % R
+    b = R1csBuilder()
+    prev, cur = b.public(f0), b.public(f1)
+    expected = b.public()
+    pv, cv = f0, f1
+    for _ in range(2, steps + 1):
+        nv = (pv + cv) % R
+        nxt = b.private(nv)
+        b.constrain(b.lc((nxt, 1)), b.lc((prev, 1), (cur, 1)))
+        pv, cv = cv, nv
+        prev, cur = cur, nxt
+    b.values[expected] = cv
+    b.constrain(b.lc((expected, 1)), b.lc((cur, 1)))
+    return b.finish()
+
+
+class PoseidonParams:
+    """Width 3, x^5 S-box, 8 full and 57 partial rounds (the reference's shape,
+    common/include/crypto/poseidon.hpp:19-23) with seeded constants: 195 round
+    constants and a 3x3 MDS matrix drawn as uniform Fr from mt19937_64(seed).
+    (The reference's own constants come from its Grain LFSR generator; these
+    are a stand-in of the same shape for synthetic workloads.)"""
+    WIDTH, FULL, PARTIAL = 3, 8, 57
+
+    def __init__(self, seed: int = 0x9053):
+        rng = MT64(seed)
+        rinv = pow(MONT, -1, R)
+        draw = lambda: fr_random_mont(rng) * rinv % R
+        self.rc = [draw() for _ in range(self.WIDTH * (self.FULL + self.PARTIAL))]
+        self.mds = [[draw() for _ in range(3)] for _ in range(3)]
+
+    def permute(self, state):
+        s = list(state)
+        k = 0
+        for rnd in range(self.FULL + self.PARTIAL):
+            s = [(x + self.rc[k + i]) % R for i, x in enumerate(s)]
+            k += 3
+            full = rnd < self.FULL // 2 or rnd >= self.FULL // 2 + self.PARTIAL
+            s = [pow(x, 5, R) if (full or i == 0) else x for i, x in enumerate(s)]
+            s = [sum(self.mds[r][c] * s[c] for c in range(3)) % R for r in range(3)]
+        return s
+
+    def hash2(self, left, right):
+        return self.permute([0, left, right])[0]
+
+
+def _poseidon_hash_constraints(b: R1csBuilder, prm: PoseidonParams, left, lv, right, rv, digest, folded: bool):
+    """add_hash2_constraints (protocols/groth16/common/include/r1cs/poseidon.hpp:
+    33-105): per round, state += constants; S-boxes as squared = s*s,
+    fourth = squared^2, fifth = fourth*s; then one private output and one
+    linear constraint per MDS row.  folded=True keeps the MDS rows as linear
+    combinations instead (243 + 1 constraints per hash instead of 439)."""
+    state = [b.lc(const=0), b.lc((left, 1)), b.lc((right, 1))]
+    vals = [0, lv, rv]
+    k = 0
+    for rnd in range(prm.FULL + prm.PARTIAL):
+        sboxed, sv = list(state), list(vals)
+        for i in range(3):
+            state[i] = b.add(state[i], b.lc(const=prm.rc[k]))
+            vals[i] = (vals[i] + prm.rc[k]) % R
+            k += 1
+        full = rnd < prm.FULL // 2 or rnd >= prm.FULL // 2 + prm.PARTIAL
+        nsb = 3 if full else 1
+        for i in range(nsb):
+            x = vals[i]
+            x2 = x * x % R
+            x4 = x2 * x2 % R
+            x5 = x4 * x % R
+            sq, fo, fi = b.private(x2), b.private(x4), b.private(x5)
+            b.constrain(b.lc((sq, 1)), state[i], state[i])
+            b.constrain(b.lc((fo, 1)), b.lc((sq, 1)), b.lc((sq, 1)))
+            b.constrain(b.lc((fi, 1)), b.lc((fo, 1)), state[i])
+            sboxed[i], sv[i] = b.lc((fi, 1)), x5
+        for i in range(nsb, 3):
+            sboxed[i], sv[i] = state[i], vals[i]
+        nxt, nv = [], []
+        for r in range(3):
+            mixed = b.lc(const=0)
+            for c in range(3):
+                mixed = b.add(mixed, b.scale(sboxed[c], prm.mds[r][c]))
+            v = sum(prm.mds[r][c] * sv[c] for c in range(3)) % R
+            if folded:
+                nxt.append(mixed)
+            else:
+                out = b.private(v)
+                b.constrain(b.lc((out, 1)), mixed)
+                nxt.append(b.lc((out, 1)))
+            nv.append(v)
+        state, vals = nxt, nv
+    b.constrain(b.lc((digest, 1)), state[0])
+
+
+def poseidon(count: int, seed: int, density: str = "full", folded: bool = False,
+             params: PoseidonParams | None = None) -> R1cs:
+    """build_poseidon (protocols/groth16/app/gen.cpp:136-162): count
+    independent hashes with a public digest and private preimage (left, right),
+    allocated per hash in the order digest, left, right, then the hash's
+    intermediates.  Preimages uniform Fr from mt19937_64(seed ^ 0x9E5), with
+    the density's zero positions from seeded_mask(2 * count, ..., seed ^ 0x3A5C2)."""
+    prm = params or PoseidonParams()
+    n_in = 2 * count
+    nz = density_nonzero(n_in, density)
+    mask = seeded_mask(n_in, nz, seed ^ 0x3A5C2) if nz < n_in else [1] * n_in
+    rng = MT64(seed ^ 0x9E5)
+    rinv = pow(MONT, -1, R)
+    inputs = []
+    for i in range(n_in):
+        v = 0
+        if mask[i]:
+            while v == 0:
+                v = fr_random_mont(rng) * rinv % R
+        inputs.append(v)
+    b = R1csBuilder()
+    for h in range(count):
+        lv, rv = inputs[2 * h], inputs[2 * h + 1]
+        digest = b.public(prm.hash2(lv, rv))
+        left, right = b.private(lv), b.private(rv)
+        _poseidon_hash_constraints(b, prm, left, lv, right, rv, digest, folded)
+    return b.finish()
+
+
 def random_scalars(n: int, seed: int) -> np.ndarray:
     """Uniform Fr Montgomery residues (libff rejection rule) from numpy's PCG64,
     for synthetic proving keys and operator inputs."""

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/nkr_g16.h"
+#include "keygen.h"
 #include "msm.h"
 #include "ntt.h"
 #include "pairing.h"
@@ -246,6 +247,18 @@ int nkr_g16_blind(const nkr_g16_pk *pk, const void *raw, const void *r, const vo
         std::memcpy(&rr, r, sizeof(rr));
         std::memcpy(&ss, s, sizeof(ss));
         groth16_blind(*pk, raw, rr, ss, proof_out);
+    });
+}
+
+int nkr_g16_setup(const nkr_g16_circuit *circuit, const void *trapdoors, int device, const nkr_g16_pk_out *pk,
+                  const nkr_g16_vk_out *vk)
+{
+    return guarded([&] {
+        NKR_REQUIRE(circuit && trapdoors && pk && vk, "null argument");
+        set_device(device);
+        Trapdoors td;
+        std::memcpy(&td, trapdoors, sizeof(td));
+        groth16_keygen(*circuit, td, device, *pk, *vk);
     });
 }
 

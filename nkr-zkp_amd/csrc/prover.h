@@ -20,6 +20,9 @@
 
 namespace nkr {
 
+// validate_circuit's preconditions on a circuit view; throws Error.
+void check_circuit(const nkr_g16_circuit &c);
+
 class Groth16Prover {
 public:
     Groth16Prover(const nkr_g16_circuit &circuit, const nkr_g16_pk &pk, int device);

@@ -43,10 +43,9 @@ void upload(DeviceBuffer<T> &buf, const void *src, size_t count, hipStream_t str
 
 }  // namespace
 
-Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int device) : device_(device)
+void check_circuit(const nkr_g16_circuit &c)
 {
-    // Same preconditions as validate_circuit / validate_proving_key
-    // (protocols/groth16/common/src/protocol.cpp:9-67).
+    // Same preconditions as validate_circuit (protocols/groth16/common/src/protocol.cpp:9-52).
     NKR_REQUIRE(c.rows >= 2 && is_pow2(c.rows), "constraint count must be a power of two (>= 2)");
     NKR_REQUIRE(c.rows <= (1ull << 28), "constraint count exceeds the 2-adicity of Fr");
     NKR_REQUIRE(c.cols >= 1 && c.cols < (1ull << 31), "invalid variable count");
@@ -54,6 +53,14 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     check_csr(c.rows, c.cols, c.a_row_ptr, c.a_col_idx, c.a_values, c.a_nnz, "A");
     check_csr(c.rows, c.cols, c.b_row_ptr, c.b_col_idx, c.b_values, c.b_nnz, "B");
     check_csr(c.rows, c.cols, c.c_row_ptr, c.c_col_idx, c.c_values, c.c_nnz, "C");
+    const uint64_t nnz[3] = {c.a_nnz, c.b_nnz, c.c_nnz};
+    for (int m = 0; m < 3; ++m) NKR_REQUIRE(nnz[m] < (1ull << 32), "matrix has too many non-zeros");
+}
+
+Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int device) : device_(device)
+{
+    // validate_circuit / validate_proving_key (protocols/groth16/common/src/protocol.cpp:9-67)
+    check_circuit(c);
     NKR_REQUIRE(pk.a_query && pk.b_g1_query && pk.b_g2_query && pk.l_query && pk.h_query,
                 "proving key query vectors missing");
 

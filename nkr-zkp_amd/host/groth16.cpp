@@ -66,9 +66,13 @@ public:
     }
     const char *name() const override { return "hip"; }
 
-    void setup(const Circuit &, ProvingKey &, VerificationKey &) override
+    // Trapdoors from std::random_device, as the CPU backend draws them
+    // (backend.cpp:48-52); setup_with_trapdoors takes them from the caller.
+    void setup(const Circuit &circuit, ProvingKey &pk, VerificationKey &vk) override
     {
-        throw std::runtime_error("key generation is not part of this backend; generate keys with the cpu backend");
+        std::random_device rd;
+        const uint64_t seed = (uint64_t(rd()) << 32) ^ rd();
+        setup_with_trapdoors(circuit, sample_trapdoors(seed), pk, vk, device_);
     }
 
     void prepare(const Circuit &circuit, const ProvingKey &key) override
@@ -126,14 +130,40 @@ bool verify(const VerificationKey &key, const std::vector<Scalar> &public_input,
     return accepted != 0;
 }
 
-std::array<Scalar, 2> sample_blinding(uint64_t seed)
+template <size_t N>
+std::array<Scalar, N> sample_scalars(uint64_t seed);
+
+std::array<Scalar, 2> sample_blinding(uint64_t seed) { return sample_scalars<2>(seed); }
+
+std::array<Scalar, 5> sample_trapdoors(uint64_t seed) { return sample_scalars<5>(seed); }
+
+void setup_with_trapdoors(const Circuit &circuit, const std::array<Scalar, 5> &trapdoors, ProvingKey &pk,
+                          VerificationKey &vk, int device)
+{
+    const size_t rows = circuit.a.num_rows, cols = circuit.a.num_cols;
+    pk.a_query.resize(cols);
+    pk.b_g1_query.resize(cols);
+    pk.b_g2_query.resize(cols);
+    pk.l_query.resize(cols);
+    pk.h_query.resize(rows);
+    vk.ic.resize(circuit.public_inputs + 1);
+    const nkr_g16_pk_out po{pk.a_query.data(), pk.b_g1_query.data(), pk.b_g2_query.data(), pk.l_query.data(),
+                            pk.h_query.data(),  &pk.alpha_g1,          &pk.beta_g1,           &pk.beta_g2,
+                            &pk.delta_g1,       &pk.delta_g2};
+    const nkr_g16_vk_out vo{&vk.alpha_g1_beta_g2, &vk.gamma_g2, &vk.delta_g2, vk.ic.data()};
+    const nkr_g16_circuit cv = circuit_view(circuit);
+    check(nkr_g16_setup(&cv, trapdoors.data(), device, &po, &vo), "setup");
+}
+
+template <size_t N>
+std::array<Scalar, N> sample_scalars(uint64_t seed)
 {
     // Fp_model::random_element (libff fp.tcc:719-744): four words, bits above
     // the modulus MSB cleared, rejected while >= r.
     static const uint64_t R[4] = {0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull,
                                   0x30644e72e131a029ull};
     std::mt19937_64 rng(seed);
-    std::array<Scalar, 2> out{};
+    std::array<Scalar, N> out{};
     for (Scalar &x : out) {
         for (;;) {
             for (uint64_t &w : x.mont) w = rng();

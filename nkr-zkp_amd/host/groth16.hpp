@@ -100,6 +100,11 @@ std::vector<std::string> available_backends();
 Proof blind_proof(const ProvingKey &key, const RawProof &raw, const Scalar &r, const Scalar &s);
 // Seeded r, s by libff's random_element rule on mt19937_64(seed).
 std::array<Scalar, 2> sample_blinding(uint64_t seed);
+// Seeded t, alpha, beta, gamma, delta (the order backend.cpp:48-52 draws them).
+std::array<Scalar, 5> sample_trapdoors(uint64_t seed);
+// Backend::setup with given trapdoors, on the GPU (nkr_g16_setup).
+void setup_with_trapdoors(const Circuit &circuit, const std::array<Scalar, 5> &trapdoors, ProvingKey &pk,
+                          VerificationKey &vk, int device = 0);
 // verify (protocol.cpp:96-108): the pairing check on the host.
 bool verify(const VerificationKey &key, const std::vector<Scalar> &public_input, const Proof &proof);
 

@@ -39,6 +39,16 @@ class Pk(ctypes.Structure):
         "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2")]
 
 
+class PkOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
+        "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2")]
+
+
+class VkOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("alpha_g1_beta_g2", "gamma_g2", "delta_g2", "ic")]
+
+
 class Vk(ctypes.Structure):
     _fields_ = [("alpha_g1_beta_g2", ctypes.c_void_p), ("gamma_g2", ctypes.c_void_p),
                 ("delta_g2", ctypes.c_void_p), ("ic", ctypes.c_void_p), ("ic_len", ctypes.c_uint64)]
@@ -95,6 +105,7 @@ def lib() -> ctypes.CDLL:
             "nkr_memcpy_dtoh": [vp, vp, u64],
             "nkr_device_synchronize": [],
             "nkr_g16_verify": [ctypes.POINTER(Vk), vp, u64, vp, ctypes.POINTER(ctypes.c_int)],
+            "nkr_g16_setup": [ctypes.POINTER(Circuit), vp, i32, ctypes.POINTER(PkOut), ctypes.POINTER(VkOut)],
             "nkr_bn254_pairing": [vp, vp, vp],
             "nkr_bn254_pairing_product": [vp, vp, u64, vp],
             "nkr_bn254_miller_loop": [vp, vp, vp],
@@ -243,27 +254,57 @@ def msm(bases, scalars, g2: bool = False, device: int = 0) -> np.ndarray:
     return out
 
 
+def _circuit_struct(circ, keep: list) -> Circuit:
+    """Circuit view of an object with rows, cols, public_inputs and a, b, c =
+    (row_ptr, col_idx, values); the arrays it points at are appended to keep."""
+    def arr(x, dtype, width=None):
+        a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
+        if width:
+            a = a.reshape(-1, width)
+        keep.append(a)
+        return a
+
+    mats = []
+    for m in (circ.a, circ.b, circ.c):
+        mats.append((arr(m[0], np.uint32), arr(m[1], np.uint32), arr(m[2], np.uint64, 4)))
+    return Circuit(circ.rows, circ.cols, circ.public_inputs,
+                   *[v for (rp, ci, va) in mats for v in (_p(rp), _p(ci) if ci.size else None,
+                                                           _p(va) if va.size else None, va.shape[0])])
+
+
+class Keys:
+    """Proving and verification key arrays (uint64 words, reference layouts)."""
+
+
+def setup(circ, trapdoors, device: int = 0) -> Keys:
+    """Backend::setup with caller-supplied trapdoors (t, alpha, beta, gamma,
+    delta as a (5, 4) Montgomery array): one object holding the proving key
+    fields (a_query, ..., delta_g2) and the verification key fields
+    (alpha_g1_beta_g2, gamma_g2, ic; delta_g2 is shared)."""
+    keep = []
+    c = _circuit_struct(circ, keep)
+    td = _words(trapdoors, 4)
+    assert td.shape[0] == 5
+    k = Keys()
+    rows, cols, pub = circ.rows, circ.cols, circ.public_inputs
+    shapes = {"a_query": (cols, 12), "b_g1_query": (cols, 12), "b_g2_query": (cols, 24), "l_query": (cols, 12),
+              "h_query": (rows, 12), "alpha_g1": (12,), "beta_g1": (12,), "beta_g2": (24,), "delta_g1": (12,),
+              "delta_g2": (24,), "alpha_g1_beta_g2": (GT_WORDS,), "gamma_g2": (24,), "ic": (pub + 1, 12)}
+    for name, shape in shapes.items():
+        setattr(k, name, np.zeros(shape, dtype=np.uint64))
+    pk_out = PkOut(*[_p(getattr(k, n)) for n, _ in PkOut._fields_])
+    vk_out = VkOut(_p(k.alpha_g1_beta_g2), _p(k.gamma_g2), _p(k.delta_g2), _p(k.ic))
+    _check(lib().nkr_g16_setup(ctypes.byref(c), _p(td), device, ctypes.byref(pk_out), ctypes.byref(vk_out)))
+    return k
+
+
 class Prover:
     """Groth16 backend context: prepare in the constructor, prove per witness."""
 
     def __init__(self, circ, pk, device: int = 0):
         self._keep = []
-
-        def arr(x, dtype, width=None):
-            a = np.ascontiguousarray(np.asarray(x, dtype=dtype))
-            if width:
-                a = a.reshape(-1, width)
-            self._keep.append(a)
-            return a
-
-        mats = []
-        for m in (circ.a, circ.b, circ.c):
-            rp, ci, va = arr(m[0], np.uint32), arr(m[1], np.uint32), arr(m[2], np.uint64, 4)
-            mats.append((rp, ci, va))
         self.rows, self.cols = circ.rows, circ.cols
-        c = Circuit(circ.rows, circ.cols, circ.public_inputs,
-                    *[v for (rp, ci, va) in mats for v in (_p(rp), _p(ci) if ci.size else None,
-                                                            _p(va) if va.size else None, va.shape[0])])
+        c = _circuit_struct(circ, self._keep)
         self.pk_struct = self._pk(pk)
         h = ctypes.c_void_p()
         _check(lib().nkr_g16_ctx_create(ctypes.byref(c), ctypes.byref(self.pk_struct), device, ctypes.byref(h)))

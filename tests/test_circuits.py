@@ -41,3 +41,68 @@ def test_random_scalars_canonical():
     s = circuits.random_scalars(10000, 1)
     ints = [O.words_to_int(r) for r in s]
     assert all(v < O.R for v in ints) and len(set(ints)) == 10000
+
+
+def satisfied(c) -> bool:
+    az = O.spmv(c.rows, c.cols, *c.a, c.witness)
+    bz = O.spmv(c.rows, c.cols, *c.b, c.witness)
+    cz = O.spmv(c.rows, c.cols, *c.c, c.witness)
+    return np.array_equal(O.fr_mul(az, bz), cz)
+
+
+def test_poseidon_counts_match_reference_test():
+    """protocols/groth16/tests/poseidon_test.cpp:40-48: one hash gives 439
+    constraints, 442 variables with the constant column, one public input,
+    and a 512 x 512 padded R1CS; the witness satisfies it and a changed digest
+    does not."""
+    c = circuits.poseidon(1, 7)
+    assert c.public_inputs == 1 and c.rows == 512 and c.cols == 512
+    assert int(c.a[0][439]) == int(c.a[0][-1]) and int(c.a[0][438]) < int(c.a[0][439])
+    live_cols = int(max(c.a[1].max(), c.b[1].max(), c.c[1].max())) + 1
+    assert live_cols == 442
+    assert satisfied(c)
+    w = c.witness.copy()
+    w[1] = O.fr_add(w[1:2], O.fr_array([1]))[0]
+    c.witness = w
+    assert not satisfied(c)
+
+
+def test_poseidon_hash_matches_permutation_model():
+    """The digest column holds hash2(left, right) = permute([0, left, right])[0]
+    of an independent restatement of the round schedule (4 full, 57 partial, 4 full)."""
+    prm = circuits.PoseidonParams()
+    left, right = 1, 2
+    s = [0, left, right]
+    k = 0
+    for rnd in range(65):
+        s = [(x + prm.rc[k + i]) % O.R for i, x in enumerate(s)]
+        k += 3
+        if 4 <= rnd < 61:
+            s[0] = pow(s[0], 5, O.R)
+        else:
+            s = [pow(x, 5, O.R) for x in s]
+        s = [sum(prm.mds[r][cc] * s[cc] for cc in range(3)) % O.R for r in range(3)]
+    assert prm.hash2(left, right) == s[0]
+
+
+@pytest.mark.parametrize("folded", [False, True])
+def test_poseidon_batch_satisfies(folded):
+    c = circuits.poseidon(5, 11, density="hybrid", folded=folded)
+    per_hash = 244 if folded else 439
+    assert int(c.a[0][5 * per_hash]) == int(c.a[0][-1])
+    assert c.public_inputs == 5 and satisfied(c)
+
+
+def test_fibonacci():
+    c = circuits.fibonacci(1024, 3)
+    assert c.rows == 1024 and c.cols == 2048 and c.public_inputs == 3
+    assert satisfied(c)
+    f = O.fr_ints(c.witness[:4])
+    assert f[0] == 1
+    # F_steps (public column 3) is the last private term
+    last = O.fr_ints(c.witness[3 + 1023:3 + 1024])[0]
+    assert f[3] == last
+    a, b = f[1], f[2]
+    for _ in range(1023):
+        a, b = b, (a + b) % O.R
+    assert f[3] == b
