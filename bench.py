@@ -135,6 +135,9 @@ def main() -> int:
     ap.add_argument("--density", default="sparse")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mix")
+    ap.add_argument("--batch-per-shape", type=int, default=4, help="witnesses per circuit shape in the mix")
+    ap.add_argument("--batch-reps", type=int, default=3)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -245,20 +248,84 @@ def main() -> int:
                      "frac": round(achieved_mul / FQ_MUL_PEAK, 4)},
         },
     }
+    h = None
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline sample")
-        h = np.zeros((circ.rows, 4), dtype=np.uint64)
         _, (_, _, _, h) = prover.prove(wit, intermediates=True)
-        result["cpu_baseline"] = cpu_baseline(circ, pk, wit, h, args.cpu_budget)
-    print(json.dumps(result), flush=True)
     N._check(N.lib().nkr_dev_free(dptr))
     prover.close()
+    if not args.no_batch:
+        result["batch"] = batch_leg(args.batch_per_shape, args.batch_reps)
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline sample")
+        result["cpu_baseline"] = cpu_baseline(circ, pk, wit, h, args.cpu_budget)
+    print(json.dumps(result), flush=True)
     return 0
 
 
 def _msm_window(n: int) -> int:
+    """MsmPlan::choose: ~3/4 log2 n in [4, 15], moved to a neighbour when the
+    top window would keep fewer than 4 of the scalar's 254 bits."""
     lg = max(0, (n - 1).bit_length())
-    return max(4, min(15, (lg * 3 + 2) // 4))
+    c = max(4, min(15, (lg * 3 + 2) // 4))
+    top = lambda cc: 254 - cc * ((255 + cc - 1) // cc - 1)
+    if top(c) < 4:
+        if c < 15 and top(c + 1) >= 4:
+            c += 1
+        elif c > 4 and top(c - 1) >= 4:
+            c -= 1
+    return c
+
+
+def batch_leg(per_shape: int, reps: int) -> dict:
+    """Batched vs plain proofs/s over the heterogeneous mix (16 circuit shapes,
+    domains 2^9 .. 2^18, per_shape witnesses each).  Plain: one
+    nkr_g16_prove_blind per proof, back to back.  Batched: the same proofs in
+    one nkr_g16_batch_prove call.  Both time host-resident witnesses in and
+    blinded proofs out; every batched proof must equal its plain proof bit for
+    bit and verify."""
+    log(f"batch mix: generating 16 circuit shapes x {per_shape} witnesses")
+    t0 = time.perf_counter()
+    mix = circuits.batch_mix(SEED, per_shape)
+    gen_s = time.perf_counter() - t0
+    items, keyed = [], []
+    for k, (name, circ, wits) in enumerate(mix):
+        keys = N.setup(circ, circuits.random_scalars(5, SEED + 17 * k))
+        pv = N.Prover(circ, keys)
+        keyed.append((pv, keys))
+        for j, w in enumerate(wits):
+            rs = circuits.random_scalars(2, SEED ^ (k << 8) ^ j)
+            items.append((name, circ, pv, keys, np.ascontiguousarray(w), rs[0].copy(), rs[1].copy()))
+    log(f"batch mix: {len(items)} proofs ready ({gen_s:.1f}s to generate), rows 2^"
+        f"{min(c.rows for _, c, *_ in items).bit_length() - 1} .. 2^{max(c.rows for _, c, *_ in items).bit_length() - 1}")
+
+    plain_out = [pv.prove_blind(w, r, s) for (_, _, pv, _, w, r, s) in items]  # warm-up + reference results
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for (_, _, pv, _, w, r, s) in items:
+            pv.prove_blind(w, r, s)
+    plain_s = (time.perf_counter() - t0) / reps
+
+    batch = N.Batch()
+    args = [(pv, w, r, s) for (_, _, pv, _, w, r, s) in items]
+    outs = batch.prove(args)  # warm-up (work buffers grow here)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs = batch.prove(args)
+    batch_s = (time.perf_counter() - t0) / reps
+    exact = all(np.array_equal(o[0], p[0]) and np.array_equal(o[1], p[1]) for o, p in zip(outs, plain_out))
+    verified = all(N.verify(keys, w[1:1 + circ.public_inputs], o[1])
+                   for (_, circ, _, keys, w, _, _), o in zip(items, outs))
+    batch.close()
+    for pv, _ in keyed:
+        pv.close()
+    n = len(items)
+    log(f"batch mix: plain {1000 * plain_s:.1f} ms, batched {1000 * batch_s:.1f} ms for {n} proofs; "
+        f"bit-exact={exact} verified={verified}")
+    return {"proofs": n, "shapes": [name for name, _, _ in mix], "per_shape": per_shape,
+            "plain_ms": round(1000 * plain_s, 3), "batched_ms": round(1000 * batch_s, 3),
+            "plain_proofs_per_s": round(n / plain_s, 2), "batched_proofs_per_s": round(n / batch_s, 2),
+            "speedup": round(plain_s / batch_s, 3), "bit_exact_vs_plain": exact, "all_verified": verified,
+            "reps": reps}
 
 
 if __name__ == "__main__":

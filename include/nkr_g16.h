@@ -110,6 +110,29 @@ int nkr_g16_prove_blind(nkr_g16_ctx *ctx, const void *witness, const void *r, co
 int nkr_g16_prove_blind_dev(nkr_g16_ctx *ctx, const void *witness_dev, const void *r, const void *s,
                             void *raw_out, void *proof_out);
 
+/* Batched proving.  The reference proves one circuit per process and runs
+ * independent proofs side by side as separate processes
+ * (scripts/bench/runner.py:260-355); this entry point is a new caller of the
+ * same contexts: count proofs (any mix of contexts created on the batch's
+ * device, the same context any number of times) go through the GPU as one
+ * launch sequence -- one SpMV launch, one launch per NTT pass per domain
+ * size, shared MSM accumulation and reduction launches -- and each item
+ * receives exactly what nkr_g16_prove_blind gives for it alone.  Items are
+ * host memory: witness (cols Fr), r and s (Fr), raw_out (576 B), proof_out
+ * (384 B).  A batch handle owns per-proof work buffers that grow to the
+ * largest batch it has run; calls on one handle must be serialised, and no
+ * context in a batch may be used by another thread during the call. */
+typedef struct nkr_g16_batch nkr_g16_batch;
+typedef struct {
+    nkr_g16_ctx *ctx;
+    const void *witness;
+    const void *r, *s;
+    void *raw_out, *proof_out;
+} nkr_g16_batch_item;
+int nkr_g16_batch_create(int device, nkr_g16_batch **out);
+int nkr_g16_batch_prove(nkr_g16_batch *batch, const nkr_g16_batch_item *items, uint64_t count);
+int nkr_g16_batch_destroy(nkr_g16_batch *batch);
+
 /* Stage timing with HIP events on the context's stream.  While enabled,
  * every prove adds its per-stage device time (ms) to running totals:
  *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit

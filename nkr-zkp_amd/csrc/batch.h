@@ -1,0 +1,64 @@
+// Batched proving: many Groth16 proofs (different circuits and sizes, or
+// the same circuit with different witnesses) as one launch sequence.
+//
+// The plain path pays every kernel's latency once per proof; small proofs
+// (2^10 .. 2^18 constraints) are dominated by those latencies, not by work.
+// A batch runs each stage once over a work list covering every proof:
+//   SpMV        one launch over (job, matrix, row)
+//   NTT chain   one launch per pass per size class (all A, B, C of the class)
+//   quotient    one launch per size class
+//   sort        per schedule (witness and H of every proof)
+//   MSM         one MsmBatch for all witness G1 MSMs (3 per proof) and the
+//               G2 MSMs on a side stream, one for all H MSMs on the main
+//               stream: each accumulation / fold / tree level is one launch
+//               for the whole batch
+//   host tail   the r,s-only blinding terms while the GPU works, then the
+//               final sums and blinding on a pool of host threads.
+// Each proof's result is bit-identical to proving it alone.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "msm.h"
+#include "prover.h"
+
+namespace nkr {
+
+struct BatchItem {
+    Groth16Prover *ctx;
+    const void *witness;  // cols Fr, host memory
+    fr_t r, s;
+    void *raw_out;    // 576 B
+    void *proof_out;  // 384 B
+};
+
+class BatchProver {
+public:
+    explicit BatchProver(int device);
+    ~BatchProver();
+    BatchProver(const BatchProver &) = delete;
+    BatchProver &operator=(const BatchProver &) = delete;
+
+    void prove(const std::vector<BatchItem> &items);
+    int device() const { return device_; }
+
+private:
+    struct Slot {
+        DeviceBuffer<fr_t> z, pa, pb, pc;
+        MsmSchedule wsched, hsched;
+        DeviceBuffer<Xyzz<fq_t>> planes1;
+        DeviceBuffer<Xyzz<fq2_t>> planes2;
+    };
+    int device_;
+    hipStream_t stream_ = nullptr, stream2_ = nullptr;
+    hipEvent_t ev_start_ = nullptr, ev_side_done_ = nullptr;
+    std::vector<std::unique_ptr<Slot>> slots_;
+    MsmBatch<fq_t> batch_w_, batch_h_;
+    MsmBatch<fq2_t> batch_g2_;
+    PinnedStaging<uint8_t> witness_stage_, table_stage_;
+    DeviceBuffer<uint8_t> tables_;
+    PinnedStaging<uint8_t> planes_host_;
+};
+
+}  // namespace nkr

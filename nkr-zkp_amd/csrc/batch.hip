@@ -1,0 +1,225 @@
+// Batched Groth16 proving on one GPU (see batch.h for the stage plan).
+#include "batch.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <thread>
+
+namespace nkr {
+
+namespace {
+
+// fn(i) for i < n on up to 16 host threads.
+template <class Fn>
+void parallel_for(size_t n, Fn fn)
+{
+    const size_t hw = std::max<size_t>(1, std::thread::hardware_concurrency());
+    const size_t workers = std::min<size_t>({n, hw, 16});
+    if (workers <= 1) {
+        for (size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    std::exception_ptr err;
+    std::atomic<bool> failed{false};
+    for (size_t w = 0; w < workers; ++w)
+        pool.emplace_back([&] {
+            for (size_t i; (i = next.fetch_add(1)) < n;) {
+                try {
+                    fn(i);
+                } catch (...) {
+                    if (!failed.exchange(true)) err = std::current_exception();
+                }
+            }
+        });
+    for (auto &t : pool) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
+size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+}  // namespace
+
+BatchProver::BatchProver(int device) : device_(device)
+{
+    NKR_HIP(hipSetDevice(device_));
+    NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    NKR_HIP(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
+    NKR_HIP(hipEventCreateWithFlags(&ev_side_done_, hipEventDisableTiming));
+}
+
+BatchProver::~BatchProver()
+{
+    if (stream2_) {
+        (void)hipStreamSynchronize(stream2_);
+        (void)hipStreamDestroy(stream2_);
+    }
+    if (stream_) {
+        (void)hipStreamSynchronize(stream_);
+        (void)hipStreamDestroy(stream_);
+    }
+    if (ev_start_) (void)hipEventDestroy(ev_start_);
+    if (ev_side_done_) (void)hipEventDestroy(ev_side_done_);
+}
+
+void BatchProver::prove(const std::vector<BatchItem> &items)
+{
+    const size_t K = items.size();
+    if (K == 0) return;
+    NKR_REQUIRE(K <= 16384, "a batch holds at most 16384 proofs");
+    for (const BatchItem &it : items) {
+        NKR_REQUIRE(it.ctx && it.witness && it.raw_out && it.proof_out, "batch item with a null member");
+        NKR_REQUIRE(it.ctx->device_ == device_, "batch item prepared on another device");
+        NKR_REQUIRE(it.ctx->have_blind_, "the proving key given to prepare lacks alpha/beta/delta");
+    }
+    NKR_HIP(hipSetDevice(device_));
+    while (slots_.size() < K) slots_.push_back(std::make_unique<Slot>());
+
+    // witnesses -> pinned staging -> slot buffers
+    size_t wbytes = 0, pbytes = 0;
+    uint32_t max_rows = 0;
+    for (const BatchItem &it : items) {
+        const Groth16Prover &g = *it.ctx;
+        wbytes += (size_t)g.cols_ * sizeof(fr_t);
+        pbytes += (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>) + g.wplan_.c * sizeof(Xyzz<fq2_t>);
+        max_rows = std::max(max_rows, g.rows_);
+    }
+    uint8_t *wst = witness_stage_.reuse(wbytes);
+    size_t off = 0;
+    for (size_t i = 0; i < K; ++i) {
+        const Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        const size_t bytes = (size_t)g.cols_ * sizeof(fr_t);
+        std::memcpy(wst + off, items[i].witness, bytes);
+        sl.z.ensure(g.cols_);
+        sl.pa.ensure(g.rows_);
+        sl.pb.ensure(g.rows_);
+        sl.pc.ensure(g.rows_);
+        sl.planes1.ensure(3 * g.wplan_.c + g.hplan_.c);
+        sl.planes2.ensure(g.wplan_.c);
+        NKR_HIP(hipMemcpyAsync(sl.z.get(), wst + off, bytes, hipMemcpyHostToDevice, stream_));
+        off += bytes;
+    }
+    witness_stage_.uploaded(stream_);
+
+    // device tables: SpMV jobs, then per size class the transform and quotient pointer lists
+    std::map<uint32_t, std::vector<size_t>> classes;
+    for (size_t i = 0; i < K; ++i) classes[items[i].ctx->log_n_].push_back(i);
+    size_t tbytes = align16(K * sizeof(SpmvJob));
+    for (const auto &cl : classes) tbytes += align16(cl.second.size() * 8 * sizeof(fr_t *));
+    uint8_t *tst = table_stage_.reuse(tbytes);
+    tables_.ensure(tbytes);
+    SpmvJob *jobs = reinterpret_cast<SpmvJob *>(tst);
+    for (size_t i = 0; i < K; ++i) {
+        const Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        for (int m = 0; m < 3; ++m) jobs[i].m[m] = g.views_[m];
+        jobs[i].z = sl.z.get();
+        jobs[i].y[0] = sl.pa.get();
+        jobs[i].y[1] = sl.pb.get();
+        jobs[i].y[2] = sl.pc.get();
+    }
+    struct ClassTables {
+        const NttDomain *domain;
+        uint32_t count;
+        size_t all3, a_only, quot;  // byte offsets into tables_
+    };
+    std::vector<ClassTables> ctabs;
+    off = align16(K * sizeof(SpmvJob));
+    for (const auto &cl : classes) {
+        const size_t k = cl.second.size();
+        fr_t **p = reinterpret_cast<fr_t **>(tst + off);
+        ClassTables ct{items[cl.second[0]].ctx->domain_.get(), (uint32_t)k, off, off + 3 * k * sizeof(fr_t *),
+                       off + 4 * k * sizeof(fr_t *)};
+        for (size_t q = 0; q < k; ++q) {
+            Slot &sl = *slots_[cl.second[q]];
+            p[3 * q + 0] = sl.pa.get();
+            p[3 * q + 1] = sl.pb.get();
+            p[3 * q + 2] = sl.pc.get();
+            p[3 * k + q] = sl.pa.get();
+            p[4 * k + 4 * q + 0] = sl.pa.get();
+            p[4 * k + 4 * q + 1] = sl.pb.get();
+            p[4 * k + 4 * q + 2] = sl.pc.get();
+            p[4 * k + 4 * q + 3] = sl.pa.get();
+        }
+        ctabs.push_back(ct);
+        off += align16(k * 8 * sizeof(fr_t *));
+    }
+    NKR_HIP(hipMemcpyAsync(tables_.get(), tst, tbytes, hipMemcpyHostToDevice, stream_));
+    table_stage_.uploaded(stream_);
+    NKR_HIP(hipEventRecord(ev_start_, stream_));
+
+    // side stream: witness schedules, then every witness MSM of the batch
+    NKR_HIP(hipStreamWaitEvent(stream2_, ev_start_, 0));
+    std::vector<MsmJob<fq_t>> wjobs;
+    std::vector<MsmJob<fq2_t>> g2jobs;
+    for (size_t i = 0; i < K; ++i) {
+        Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        sl.wsched.build(sl.z.get(), g.wplan_, stream2_);
+        const uint32_t cw = g.wplan_.c;
+        wjobs.push_back({&g.a_, &sl.wsched, sl.planes1.get() + 0 * cw});
+        wjobs.push_back({&g.b1_, &sl.wsched, sl.planes1.get() + 1 * cw});
+        wjobs.push_back({&g.l_, &sl.wsched, sl.planes1.get() + 2 * cw});
+        g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
+    }
+    batch_w_.run(wjobs, stream2_);
+    batch_g2_.run(g2jobs, stream2_);
+    NKR_HIP(hipEventRecord(ev_side_done_, stream2_));
+
+    // main stream: A*z, B*z, C*z, the quotient chain per size class, H schedules and MSMs
+    spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, stream_);
+    for (const ClassTables &ct : ctabs) {
+        fr_t *const *all3 = reinterpret_cast<fr_t *const *>(tables_.get() + ct.all3);
+        fr_t *const *a_only = reinterpret_cast<fr_t *const *>(tables_.get() + ct.a_only);
+        fr_t *const *quot = reinterpret_cast<fr_t *const *>(tables_.get() + ct.quot);
+        ct.domain->dif_batch(all3, 3 * ct.count, true, nullptr, nullptr, nullptr, stream_);
+        ct.domain->dit_batch(all3, 3 * ct.count, false, ct.domain->coset_rev_scaled(), nullptr, stream_);
+        ct.domain->quotient_batch(quot, ct.count, stream_);
+        ct.domain->dif_batch(a_only, ct.count, true, nullptr, ct.domain->icoset_rev_scaled(), nullptr, stream_);
+    }
+    std::vector<MsmJob<fq_t>> hjobs;
+    for (size_t i = 0; i < K; ++i) {
+        Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        sl.hsched.build(sl.pa.get(), g.hplan_, stream_);
+        hjobs.push_back({&g.h_, &sl.hsched, sl.planes1.get() + 3 * g.wplan_.c});
+    }
+    batch_h_.run(hjobs, stream_);
+    NKR_HIP(hipStreamWaitEvent(stream_, ev_side_done_, 0));
+
+    // bit planes of every proof to the host
+    uint8_t *ph = planes_host_.reuse(pbytes);
+    std::vector<size_t> poff(K);
+    off = 0;
+    for (size_t i = 0; i < K; ++i) {
+        const Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        const size_t b1 = (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>);
+        const size_t b2 = g.wplan_.c * sizeof(Xyzz<fq2_t>);
+        poff[i] = off;
+        NKR_HIP(hipMemcpyAsync(ph + off, sl.planes1.get(), b1, hipMemcpyDeviceToHost, stream_));
+        NKR_HIP(hipMemcpyAsync(ph + off + b1, sl.planes2.get(), b2, hipMemcpyDeviceToHost, stream_));
+        off += b1 + b2;
+    }
+
+    // r,s-only blinding terms while the GPU works, then the host tails
+    std::vector<Groth16Prover::BlindPre> pre(K);
+    parallel_for(K, [&](size_t i) { pre[i] = items[i].ctx->blind_pre(items[i].r, items[i].s); });
+    NKR_HIP(hipStreamSynchronize(stream_));
+    parallel_for(K, [&](size_t i) {
+        const BatchItem &it = items[i];
+        const Groth16Prover &g = *it.ctx;
+        const Xyzz<fq_t> *p1 = reinterpret_cast<const Xyzz<fq_t> *>(ph + poff[i]);
+        const Xyzz<fq2_t> *p2 = reinterpret_cast<const Xyzz<fq2_t> *>(
+            ph + poff[i] + (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>));
+        const Groth16Prover::BlindJob job{it.r, it.s, it.proof_out};
+        g.finish(p1, p2, &job, &pre[i], it.raw_out);
+    });
+}
+
+}  // namespace nkr

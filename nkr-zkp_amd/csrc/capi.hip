@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/nkr_g16.h"
+#include "batch.h"
 #include "keygen.h"
 #include "msm.h"
 #include "ntt.h"
@@ -93,6 +94,10 @@ __global__ void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
 
 struct nkr_g16_ctx {
     std::unique_ptr<nkr::Groth16Prover> prover;
+};
+
+struct nkr_g16_batch {
+    std::unique_ptr<nkr::BatchProver> prover;
 };
 
 struct nkr_msm {
@@ -221,6 +226,42 @@ int nkr_g16_prove_blind_dev(nkr_g16_ctx *ctx, const void *witness_dev, const voi
                             void *raw_out, void *proof_out)
 {
     return prove_blind(ctx, witness_dev, true, r, s, raw_out, proof_out);
+}
+
+int nkr_g16_batch_create(int device, nkr_g16_batch **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(out, "null output");
+        *out = nullptr;
+        set_device(device);
+        auto b = std::make_unique<nkr_g16_batch>();
+        b->prover = std::make_unique<BatchProver>(device);
+        *out = b.release();
+    });
+}
+
+int nkr_g16_batch_prove(nkr_g16_batch *batch, const nkr_g16_batch_item *items, uint64_t count)
+{
+    return guarded([&] {
+        NKR_REQUIRE(batch && (items || count == 0), "null argument");
+        std::vector<BatchItem> v(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            const nkr_g16_batch_item &it = items[i];
+            NKR_REQUIRE(it.ctx && it.r && it.s, "batch item with a null member");
+            v[i].ctx = it.ctx->prover.get();
+            v[i].witness = it.witness;
+            std::memcpy(&v[i].r, it.r, sizeof(fr_t));
+            std::memcpy(&v[i].s, it.s, sizeof(fr_t));
+            v[i].raw_out = it.raw_out;
+            v[i].proof_out = it.proof_out;
+        }
+        batch->prover->prove(v);
+    });
+}
+
+int nkr_g16_batch_destroy(nkr_g16_batch *batch)
+{
+    return guarded([&] { delete batch; });
 }
 
 int nkr_g16_profile(nkr_g16_ctx *ctx, int enable)

@@ -37,16 +37,20 @@
 namespace nkr {
 
 struct MsmPlan {
-    uint32_t n = 0;  // number of bases / scalars
-    uint32_t c = 0;  // window width in bits
-    uint32_t W = 0;  // windows (signed digits), c*W >= 255
-    uint32_t B = 0;  // buckets 1..B, B = 2^(c-1)
+    uint32_t n = 0;      // number of bases / scalars
+    uint32_t c = 0;      // window width in bits
+    uint32_t W = 0;      // windows (signed digits), c*W >= 255
+    uint32_t B = 0;      // buckets 1..B, B = 2^(c-1)
+    uint32_t piece = 0;  // entries summed serially by one accumulation thread
     static MsmPlan choose(uint32_t n, uint32_t c_override = 0);
     uint64_t max_entries() const { return (uint64_t)n * W; }
 };
 
-// Entries per accumulation thread, fold fan-in and the completion limit.
+// Largest accumulation piece, fold fan-in and the completion limit.  Small
+// MSMs use shorter pieces (down to kMsmPieceMin) so that the serial chain of
+// one thread, not the amount of work, stops dominating their latency.
 constexpr uint32_t kMsmPiece = 64;
+constexpr uint32_t kMsmPieceMin = 4;
 constexpr uint32_t kMsmFoldIn = 16;
 constexpr uint32_t kMsmCompleteMax = 256;
 
@@ -74,7 +78,7 @@ class MsmSchedule {
 public:
     void build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer = nullptr);
     const MsmPlan &plan() const { return plan_; }
-    uint64_t max_pieces() const { return plan_.max_entries() / kMsmPiece + plan_.B + 1; }
+    uint64_t max_pieces() const { return plan_.max_entries() / plan_.piece + plan_.B + 1; }
     // Device counters: [0] entries (non-zero digits), [1] level-0 pieces.
     const uint32_t *totals() const { return bucket_start_.get() + plan_.B; }
 
@@ -111,6 +115,8 @@ private:
     std::vector<uint32_t> thread_base_;
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;
+    PinnedStaging<uint8_t> jobs_stage_;
+    PinnedStaging<uint32_t> base_stage_;
     DeviceBuffer<Xyzz<F>> pool_, sums_, tree_[2];
     DeviceBuffer<uint32_t> reg_start_, reg_count_, fold_start_;
 };

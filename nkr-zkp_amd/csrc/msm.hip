@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 namespace nkr {
 
@@ -10,6 +11,7 @@ namespace {
 
 constexpr uint32_t kEntryMask = 0x7fffffffu;
 constexpr uint32_t kSortBlocksMax = 1024;
+constexpr uint32_t kTreeTailThreads = 256;
 
 template <class F>
 struct JobDev {
@@ -25,7 +27,7 @@ struct JobDev {
     Xyzz<F> *sums;                 // B: S_{k+1}
     Xyzz<F> *tree[2];              // B each
     Xyzz<F> *planes;               // c
-    uint32_t B, c;
+    uint32_t B, c, piece;
 };
 
 // ------------------------------------------------------------ digit recoding
@@ -160,13 +162,13 @@ __device__ void block_scan(const uint32_t *__restrict__ src, uint32_t len, uint3
     __syncthreads();
 }
 
-// Bucket starts (mode 0 over counts) and level-0 piece starts (ceil(count/64)).
+// Bucket starts (mode 0 over counts) and level-0 piece starts (ceil(count/piece)).
 __global__ __launch_bounds__(1024) void k_schedule_scan(const uint32_t *__restrict__ count, uint32_t B,
-                                                        uint32_t *__restrict__ bucket_start,
+                                                        uint32_t piece, uint32_t *__restrict__ bucket_start,
                                                         uint32_t *__restrict__ piece_start)
 {
     block_scan(count, B, 0, 1, 0, bucket_start);
-    block_scan(count, B, 2, kMsmPiece, 0, piece_start, bucket_start + B + 2);
+    block_scan(count, B, 2, piece, 0, piece_start, bucket_start + B + 2);
     if (threadIdx.x == 0) bucket_start[B + 1] = piece_start[B];
 }
 
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(256) void k_regions_init(const JobDev<F> *__restric
     jb.reg_count[k] = jb.piece_start[k + 1] - jb.piece_start[k];
 }
 
-// Level 0 over all jobs: one thread sums up to kMsmPiece signed bases of one bucket.
+// Level 0 over all jobs: one thread sums up to `piece` signed bases of one bucket.
 template <class F>
 __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
                                                     const uint32_t *__restrict__ thread_base, uint32_t J)
@@ -207,8 +209,8 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
     if (p >= jb.piece_start[jb.B]) return;
     const uint32_t k = find_segment(jb.piece_start, jb.B, p);
     const uint32_t q = p - jb.piece_start[k];
-    const uint32_t beg = jb.bucket_start[k] + q * kMsmPiece;
-    const uint32_t end = min(beg + kMsmPiece, jb.bucket_start[k + 1]);
+    const uint32_t beg = jb.bucket_start[k] + q * jb.piece;
+    const uint32_t end = min(beg + jb.piece, jb.bucket_start[k + 1]);
     const Affine<F> *__restrict__ bases = jb.bases;
     const uint32_t *__restrict__ sorted = jb.sorted;
     Xyzz<F> acc = Xyzz<F>::infinity();
@@ -335,9 +337,11 @@ __device__ __forceinline__ void tree_merge(const JobDev<F> &jb, uint32_t t, uint
 }
 
 // The remaining tree levels t0 .. c-2 of one job in one workgroup (blockIdx.x
-// = job), with a barrier between levels instead of a launch.
+// = job), with a barrier between levels instead of a launch.  256 threads: a
+// G2 addition needs more than the 128 registers a 1024-thread workgroup
+// leaves per lane, and spilling it costs more than the launches it saves.
 template <class F>
-__global__ __launch_bounds__(1024) void k_tree_tail(const JobDev<F> *__restrict__ jobs, uint32_t t0)
+__global__ __launch_bounds__(kTreeTailThreads) void k_tree_tail(const JobDev<F> *__restrict__ jobs, uint32_t t0)
 {
     const JobDev<F> &jb = jobs[blockIdx.x];
     for (uint32_t t = t0; t + 1 <= jb.c - 1; ++t) {
@@ -384,11 +388,25 @@ MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
         const unsigned lg = ceil_log2(n);
         // ~3/4 log2 n, clamped so the bucket histogram fits 64 KiB of LDS.
         c = std::max(4u, std::min(15u, (lg * 3 + 2) / 4));
+        // Scalars have 254 bits; a top window with only a few of them sends a
+        // whole window's entries to a handful of buckets (long serial chains),
+        // so move to a neighbouring width whose top window keeps >= 4 bits.
+        auto top_bits = [](uint32_t cc) { return 254 - cc * ((255 + cc - 1) / cc - 1); };
+        if (top_bits(c) < 4) {
+            if (c < 15 && top_bits(c + 1) >= 4)
+                c += 1;
+            else if (c > 4 && top_bits(c - 1) >= 4)
+                c -= 1;
+        }
     }
     NKR_REQUIRE(c >= 2 && c <= 15, "MSM window must be 2..15 bits");
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.B = 1u << (c - 1);
+    // about 2^17 accumulation threads when the entries allow it
+    const uint64_t want = p.max_entries() >> 17;
+    p.piece = kMsmPieceMin;
+    while (p.piece < kMsmPiece && p.piece < want) p.piece <<= 1;
     NKR_REQUIRE((uint64_t)p.W * n < (1ull << 31), "MSM too large for 31-bit entry indices");
     return p;
 }
@@ -422,7 +440,7 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
     k_hist<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, count_.get(),
                                           block_base_.get());
     NKR_LAUNCH_CHECK();
-    k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, bucket_start_.get(), piece_start_.get());
+    k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, plan.piece, bucket_start_.get(), piece_start_.get());
     NKR_LAUNCH_CHECK();
     k_scatter<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, bucket_start_.get(),
                                              block_base_.get(), sorted_.get());
@@ -462,15 +480,17 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
     }
     NKR_REQUIRE(total < (1ull << 32), "MSM partial pool exceeds 32-bit indexing");
     pool_per_job_ = total;
-    pool_.allocate(pool_per_job_ * J);
-    sums_.allocate((size_t)Bmax * J);
-    tree_[0].allocate((size_t)Bmax * J);
-    tree_[1].allocate((size_t)Bmax * J);
-    reg_start_.allocate((size_t)Bmax * J);
-    reg_count_.allocate((size_t)Bmax * J);
-    fold_start_.allocate((size_t)(Bmax + 1) * J);
+    // grow-only: a batch whose composition changes from call to call reuses them
+    pool_.ensure(pool_per_job_ * J);
+    sums_.ensure((size_t)Bmax * J);
+    tree_[0].ensure((size_t)Bmax * J);
+    tree_[1].ensure((size_t)Bmax * J);
+    reg_start_.ensure((size_t)Bmax * J);
+    reg_count_.ensure((size_t)Bmax * J);
+    fold_start_.ensure((size_t)(Bmax + 1) * J);
 
-    std::vector<JobDev<F>> host(J);
+    static_assert(std::is_trivially_copyable<JobDev<F>>::value, "job records are raw-copied");
+    JobDev<F> *host = reinterpret_cast<JobDev<F> *>(jobs_stage_.reuse(sizeof(JobDev<F>) * J));
     thread_base_.assign(J + 1, 0);
     for (uint32_t j = 0; j < J; ++j) {
         const MsmJob<F> &jb = jobs[j];
@@ -490,15 +510,18 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.planes = jb.planes;
         d.B = jb.sched->plan().B;
         d.c = jb.sched->plan().c;
+        d.piece = jb.sched->plan().piece;
         thread_base_[j + 1] = thread_base_[j] + (uint32_t)jb.sched->max_pieces();
     }
     thread_total_ = thread_base_[J];
-    jobs_dev_.allocate(sizeof(JobDev<F>) * J);
-    thread_base_dev_.allocate(J + 1);
-    NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host.data(), sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
-    NKR_HIP(hipMemcpyAsync(thread_base_dev_.get(), thread_base_.data(), (J + 1) * sizeof(uint32_t),
-                           hipMemcpyHostToDevice, stream));
-    NKR_HIP(hipStreamSynchronize(stream));  // host staging vectors go out of scope
+    uint32_t *tb = base_stage_.reuse(J + 1);
+    std::memcpy(tb, thread_base_.data(), (J + 1) * sizeof(uint32_t));
+    jobs_dev_.ensure(sizeof(JobDev<F>) * J);
+    thread_base_dev_.ensure(J + 1);
+    NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host, sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
+    jobs_stage_.uploaded(stream);
+    NKR_HIP(hipMemcpyAsync(thread_base_dev_.get(), tb, (J + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    base_stage_.uploaded(stream);
     key_ = key;
 }
 
@@ -543,12 +566,12 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     uint32_t t = 0;
     for (; t + 1 <= cmax - 1; ++t) {
         const uint64_t work = (uint64_t)(Bmax >> (t + 1)) * (t + 2);
-        if (work <= 1024) break;  // the rest fits one workgroup per job
+        if (work <= kTreeTailThreads) break;  // the rest fits one workgroup per job
         k_tree<F><<<dim3(div_up(work, 256), J), 256, 0, stream>>>(jd, t);
         NKR_LAUNCH_CHECK();
     }
     if (t + 1 <= cmax - 1) {
-        k_tree_tail<F><<<J, 1024, 0, stream>>>(jd, t);
+        k_tree_tail<F><<<J, kTreeTailThreads, 0, stream>>>(jd, t);
         NKR_LAUNCH_CHECK();
     }
     if (timer) timer->end(kStageFold, stream);

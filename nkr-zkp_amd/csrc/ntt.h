@@ -35,6 +35,16 @@ public:
              hipStream_t stream) const;
     void dit(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, hipStream_t stream) const;
 
+    // The same transforms on `count` arrays of this size at once (one launch
+    // per pass); batch_dev is a device array of the data pointers, and the
+    // pre / post tables are shared by all of them.
+    void dif_batch(fr_t *const *batch_dev, uint32_t count, bool inverse, const fr_t *pre, const fr_t *post,
+                   const fr_t *post_const, hipStream_t stream) const;
+    void dit_batch(fr_t *const *batch_dev, uint32_t count, bool inverse, const fr_t *pre, const fr_t *post,
+                   hipStream_t stream) const;
+    // quotient() for `count` jobs; abch_dev holds 4 device pointers per job: a, b, c, h.
+    void quotient_batch(fr_t *const *abch_dev, uint32_t count, hipStream_t stream) const;
+
     // out[i] = in[bitrev(i)] (out of place).
     void bitrev(const fr_t *in, fr_t *out, hipStream_t stream) const;
 
@@ -56,8 +66,8 @@ private:
     struct Pass {
         unsigned S, np, log_L, log_cols;
     };
-    void run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const fr_t *post, const fr_t *post_const,
-             hipStream_t stream) const;
+    void run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bool inverse, const fr_t *pre,
+             const fr_t *post, const fr_t *post_const, hipStream_t stream) const;
 
     unsigned log_n_;
     unsigned lds_log_ = 11, radix_log_ = 3;

@@ -20,6 +20,7 @@ unsigned g_radix_log = 3;
 
 struct PassArgs {
     fr_t *data;
+    fr_t *const *batch;  // when set, workgroup row blockIdx.y transforms batch[blockIdx.y]
     const fr_t *tw_lo;
     const fr_t *tw_hi;
     const fr_t *roots;
@@ -94,6 +95,7 @@ template <bool DIT, int RMAX>
 __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
 {
     extern __shared__ fr_t smem[];
+    if (a.batch) a.data = a.batch[blockIdx.y];
     const uint32_t G = 1u << a.np;
     const uint32_t cols = 1u << a.log_cols;
     const uint32_t total = G << a.log_cols;
@@ -209,6 +211,15 @@ __global__ void k_quotient(const fr_t *__restrict__ a, const fr_t *__restrict__ 
     h[i] = fp_mul(fp_sub(fp_mul(a[i], b[i]), c[i]), zinv);
 }
 
+// k_quotient for job blockIdx.y with pointers abch[4 * job + {0, 1, 2, 3}] = a, b, c, h.
+__global__ void k_quotient_batch(fr_t *const *__restrict__ abch, fr_t zinv, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fr_t *const *p = abch + 4 * blockIdx.y;
+    p[3][i] = fp_mul(fp_sub(fp_mul(p[0][i], p[1][i]), p[2][i]), zinv);
+}
+
 // 2^28-th root of unity of Fr (libff alt_bn128_init.cpp:56, as an integer).
 const uint32_t kRootOfUnity28[8] = {0x725b19f0u, 0x9bd61b6eu, 0x41112ed4u, 0x402d111eu,
                                     0x8ef62abcu, 0x00e0a7ebu, 0xa58a7e85u, 0x2a3c09f0u};
@@ -319,9 +330,10 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
     NKR_LAUNCH_CHECK();
 }
 
-void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const fr_t *post,
-                    const fr_t *post_const, hipStream_t stream) const
+void NttDomain::run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bool inverse, const fr_t *pre,
+                    const fr_t *post, const fr_t *post_const, hipStream_t stream) const
 {
+    NKR_REQUIRE(count >= 1 && count <= 65535, "NTT batch must hold 1 .. 65535 transforms");
     const int d = inverse ? 1 : 0;
     const int P = (int)passes_.size();
     for (int k = 0; k < P; ++k) {
@@ -329,6 +341,7 @@ void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const f
         const Pass &p = passes_[pi];
         PassArgs a{};
         a.data = data;
+        a.batch = batch;
         a.tw_lo = tw_lo_[d].get();
         a.tw_hi = tw_hi_[d].get();
         a.roots = roots_[d].get();
@@ -344,7 +357,7 @@ void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const f
         a.log_roots = log_roots_;
         a.m_lo = m_lo_;
         a.twiddle = (pi != P - 1) ? 1u : 0u;
-        const uint32_t blocks = (uint32_t)((size_t(1) << (log_n_ - p.np)) >> p.log_cols);
+        const dim3 blocks((uint32_t)((size_t(1) << (log_n_ - p.np)) >> p.log_cols), count);
         const uint32_t elems = (1u << p.np) << p.log_cols;
         const size_t lds = size_t(sizeof(fr_t)) * elems;
         const uint32_t threads = std::min(kMaxThreads, std::max(64u, elems >> radix_log_));
@@ -366,12 +379,32 @@ void NttDomain::run(fr_t *data, bool dit, bool inverse, const fr_t *pre, const f
 void NttDomain::dif(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, const fr_t *post_const,
                     hipStream_t stream) const
 {
-    run(data, false, inverse, pre, post, post_const, stream);
+    run(data, nullptr, 1, false, inverse, pre, post, post_const, stream);
 }
 
 void NttDomain::dit(fr_t *data, bool inverse, const fr_t *pre, const fr_t *post, hipStream_t stream) const
 {
-    run(data, true, inverse, pre, post, nullptr, stream);
+    run(data, nullptr, 1, true, inverse, pre, post, nullptr, stream);
+}
+
+void NttDomain::dif_batch(fr_t *const *batch_dev, uint32_t count, bool inverse, const fr_t *pre, const fr_t *post,
+                          const fr_t *post_const, hipStream_t stream) const
+{
+    run(nullptr, batch_dev, count, false, inverse, pre, post, post_const, stream);
+}
+
+void NttDomain::dit_batch(fr_t *const *batch_dev, uint32_t count, bool inverse, const fr_t *pre, const fr_t *post,
+                          hipStream_t stream) const
+{
+    run(nullptr, batch_dev, count, true, inverse, pre, post, nullptr, stream);
+}
+
+void NttDomain::quotient_batch(fr_t *const *abch_dev, uint32_t count, hipStream_t stream) const
+{
+    NKR_REQUIRE(count >= 1 && count <= 65535, "quotient batch must hold 1 .. 65535 jobs");
+    const size_t n = size();
+    k_quotient_batch<<<dim3(div_up(n, 256), count), 256, 0, stream>>>(abch_dev, zinv_, (uint32_t)n);
+    NKR_LAUNCH_CHECK();
 }
 
 void NttDomain::bitrev(const fr_t *in, fr_t *out, hipStream_t stream) const

@@ -69,11 +69,21 @@ private:
     DeviceBuffer<Xyzz<fq2_t>> planes2_;
     void *pinned_witness_ = nullptr;
 
+    DeviceBuffer<fr_t *> poly_table_;  // {pa, pb, pc} for the batched transforms
+
     struct BlindJob {
         fr_t r, s;
         void *proof_out;
     };
+    struct BlindPre {
+        g1_xyzz A0, C0;
+        g2_xyzz B0;
+    };
     void run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind = nullptr);
+    BlindPre blind_pre(const fr_t &r, const fr_t &s) const;
+    void finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
+                void *raw_out) const;
+    friend class BatchProver;
 
     // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)
     // and the key's group elements, for blinding.

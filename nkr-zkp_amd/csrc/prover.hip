@@ -135,6 +135,9 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     pa_.allocate(rows_);
     pb_.allocate(rows_);
     pc_.allocate(rows_);
+    fr_t *const table[3] = {pa_.get(), pb_.get(), pc_.get()};
+    poly_table_.allocate(3);
+    NKR_HIP(hipMemcpyAsync(poly_table_.get(), table, sizeof(table), hipMemcpyHostToDevice, stream_));
     planes1_.allocate(3 * wplan_.c + hplan_.c);
     planes2_.allocate(wplan_.c);
     NKR_HIP(hipHostMalloc(&pinned_witness_, (size_t)cols_ * sizeof(fr_t), hipHostMallocDefault));
@@ -240,6 +243,50 @@ Xyzz<F> straus2(const Xyzz<F> &P, const fr_t &a_mont, const Xyzz<F> &Q, const fr
 
 }  // namespace
 
+// A0 = alpha_1 + r delta_1, B0 = beta_2 + s delta_2 and
+// C0 = s alpha_1 + r beta_1 + (r s) delta_1, using
+// s*A + r*B_1 - (r s) delta_1 = s alpha_1 + s ar + r beta_1 + r bs1 + (r s) delta_1.
+Groth16Prover::BlindPre Groth16Prover::blind_pre(const fr_t &r, const fr_t &s) const
+{
+    BlindPre p;
+    p.A0 = xyzz_add(jacobian_to_xyzz(alpha1_), fixed_base_mul_host(tab_delta1_, r));
+    p.B0 = xyzz_add(jacobian_to_xyzz(beta2_), fixed_base_mul_host(tab_delta2_, s));
+    p.C0 = xyzz_add(fixed_base_mul_host(tab_alpha1_, s), fixed_base_mul_host(tab_beta1_, r));
+    p.C0 = xyzz_add(p.C0, fixed_base_mul_host(tab_delta1_, fp_mul(r, s)));
+    return p;
+}
+
+// Host tail: sum_j 2^j U_j for the five MSMs, the raw proof, then the
+// blinded proof from the precomputed r,s-only terms.
+void Groth16Prover::finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
+                           void *raw_out) const
+{
+    const uint32_t cw = wplan_.c;
+    uint8_t *out = static_cast<uint8_t *>(raw_out);
+    const g1_jac ar = msm_finish(p1 + 0 * cw, cw);
+    const g1_jac bs1 = msm_finish(p1 + 1 * cw, cw);
+    const g1_jac zk = msm_finish(p1 + 2 * cw, cw);
+    const g1_jac qz = msm_finish(p1 + 3 * cw, hplan_.c);
+    const g2_jac bs2 = msm_finish(p2, cw);
+    std::memcpy(out + 0, &ar, 96);
+    std::memcpy(out + 96, &bs1, 96);
+    std::memcpy(out + 192, &zk, 96);
+    std::memcpy(out + 288, &qz, 96);
+    std::memcpy(out + 384, &bs2, 192);
+    if (!blind) return;
+    const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
+    const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(pre->A0, xar));
+    const g2_jac B = xyzz_to_jacobian_affine(xyzz_add(pre->B0, jacobian_to_xyzz(bs2)));
+    g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
+    C = xyzz_add(C, pre->C0);
+    C = xyzz_add(C, straus2(xar, blind->s, xbs1, blind->r));
+    const g1_jac Cj = xyzz_to_jacobian_affine(C);
+    uint8_t *o = static_cast<uint8_t *>(blind->proof_out);
+    std::memcpy(o, &A, 96);
+    std::memcpy(o + 96, &B, 192);
+    std::memcpy(o + 288, &Cj, 96);
+}
+
 void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind)
 {
     hipStream_t s = stream_;
@@ -266,11 +313,10 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     if (bz) NKR_HIP(hipMemcpyAsync(bz, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
     if (cz) NKR_HIP(hipMemcpyAsync(cz, pc_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
 
-    // quotient chain, no bit-reversal passes
-    fr_t *polys[3] = {pa_.get(), pb_.get(), pc_.get()};
+    // quotient chain, no bit-reversal passes; one launch per pass covers A, B and C
     if (t) t->begin(1, s);
-    for (fr_t *p : polys) domain_->dif(p, true, nullptr, nullptr, nullptr, s);
-    for (fr_t *p : polys) domain_->dit(p, false, domain_->coset_rev_scaled(), nullptr, s);
+    domain_->dif_batch(poly_table_.get(), 3, true, nullptr, nullptr, nullptr, s);
+    domain_->dit_batch(poly_table_.get(), 3, false, domain_->coset_rev_scaled(), nullptr, s);
     if (t) {
         t->end(1, s);
         t->begin(2, s);
@@ -297,46 +343,12 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
     NKR_HIP(hipMemcpyAsync(p2.data(), planes2_.get(), p2.size() * sizeof(p2[0]), hipMemcpyDeviceToHost, s));
 
-    // Blinding terms that need only r and s, while the GPU works:
-    // A0 = alpha_1 + r delta_1, B0 = beta_2 + s delta_2, and
-    // C0 = s alpha_1 + r beta_1 + (r s) delta_1, using
-    // s*A + r*B_1 - (r s) delta_1 = s alpha_1 + s ar + r beta_1 + r bs1 + (r s) delta_1.
-    g1_xyzz A0, C0;
-    g2_xyzz B0;
-    if (blind) {
-        A0 = xyzz_add(jacobian_to_xyzz(alpha1_), fixed_base_mul_host(tab_delta1_, blind->r));
-        B0 = xyzz_add(jacobian_to_xyzz(beta2_), fixed_base_mul_host(tab_delta2_, blind->s));
-        C0 = xyzz_add(fixed_base_mul_host(tab_alpha1_, blind->s), fixed_base_mul_host(tab_beta1_, blind->r));
-        C0 = xyzz_add(C0, fixed_base_mul_host(tab_delta1_, fp_mul(blind->r, blind->s)));
-    }
+    // blinding terms that need only r and s, while the GPU works
+    BlindPre pre;
+    if (blind) pre = blind_pre(blind->r, blind->s);
     NKR_HIP(hipStreamSynchronize(s));
     const auto host0 = std::chrono::steady_clock::now();
-
-    uint8_t *out = static_cast<uint8_t *>(raw_out);
-    const g1_jac ar = msm_finish(p1.data() + 0 * cw, cw);
-    const g1_jac bs1 = msm_finish(p1.data() + 1 * cw, cw);
-    const g1_jac zk = msm_finish(p1.data() + 2 * cw, cw);
-    const g1_jac qz = msm_finish(p1.data() + 3 * cw, hplan_.c);
-    const g2_jac bs2 = msm_finish(p2.data(), cw);
-    std::memcpy(out + 0, &ar, 96);
-    std::memcpy(out + 96, &bs1, 96);
-    std::memcpy(out + 192, &zk, 96);
-    std::memcpy(out + 288, &qz, 96);
-    std::memcpy(out + 384, &bs2, 192);
-
-    if (blind) {
-        const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
-        const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(A0, xar));
-        const g2_jac B = xyzz_to_jacobian_affine(xyzz_add(B0, jacobian_to_xyzz(bs2)));
-        g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
-        C = xyzz_add(C, C0);
-        C = xyzz_add(C, straus2(xar, blind->s, xbs1, blind->r));
-        const g1_jac Cj = xyzz_to_jacobian_affine(C);
-        uint8_t *o = static_cast<uint8_t *>(blind->proof_out);
-        std::memcpy(o, &A, 96);
-        std::memcpy(o + 96, &B, 192);
-        std::memcpy(o + 288, &Cj, 96);
-    }
+    finish(p1.data(), p2.data(), blind, blind ? &pre : nullptr, raw_out);
 
     if (t) {
         t->collect(stage_ms_);

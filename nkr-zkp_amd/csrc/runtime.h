@@ -91,6 +91,49 @@ private:
     size_t count_ = 0;
 };
 
+// Pinned host staging buffer for asynchronous uploads.  `reuse` waits for the
+// upload recorded by `uploaded` before the host writes the buffer again.
+template <class T>
+class PinnedStaging {
+public:
+    PinnedStaging() = default;
+    ~PinnedStaging()
+    {
+        if (event_) (void)hipEventDestroy(event_);
+        if (ptr_) (void)hipHostFree(ptr_);
+    }
+    PinnedStaging(const PinnedStaging &) = delete;
+    PinnedStaging &operator=(const PinnedStaging &) = delete;
+
+    // Room for `count` elements, writable now.
+    T *reuse(size_t count)
+    {
+        if (event_) NKR_HIP(hipEventSynchronize(event_));
+        if (count > count_) {
+            if (ptr_) NKR_HIP(hipHostFree(ptr_));
+            ptr_ = nullptr;
+            count_ = 0;
+            void *p = nullptr;
+            NKR_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+            ptr_ = static_cast<T *>(p);
+            count_ = count;
+        }
+        return ptr_;
+    }
+    // Marks the end of the uploads from the buffer on `stream`.
+    void uploaded(hipStream_t stream)
+    {
+        if (!event_) NKR_HIP(hipEventCreateWithFlags(&event_, hipEventDisableTiming));
+        NKR_HIP(hipEventRecord(event_, stream));
+    }
+    T *get() const { return ptr_; }
+
+private:
+    T *ptr_ = nullptr;
+    size_t count_ = 0;
+    hipEvent_t event_ = nullptr;
+};
+
 // Device-time accounting per stage with HIP events recorded on the stream
 // that runs the work.  Disabled timers record nothing.
 class StageTimer {

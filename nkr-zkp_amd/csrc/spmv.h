@@ -20,4 +20,12 @@ void spmv3(const CsrView &a, const CsrView &b, const CsrView &c, const fr_t *z, 
            hipStream_t stream);
 void spmv(const CsrView &a, const fr_t *z, fr_t *y, hipStream_t stream);
 
+// A*z, B*z, C*z for several circuits in one launch (jobs in device memory).
+struct SpmvJob {
+    CsrView m[3];
+    const fr_t *z;
+    fr_t *y[3];
+};
+void spmv3_batch(const SpmvJob *jobs_dev, uint32_t count, uint32_t max_rows, hipStream_t stream);
+
 }  // namespace nkr

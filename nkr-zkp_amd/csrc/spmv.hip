@@ -23,7 +23,29 @@ __global__ __launch_bounds__(256) void k_spmv3(CsrView m0, CsrView m1, CsrView m
     y[row] = acc;
 }
 
+// Job blockIdx.z, matrix blockIdx.y.
+__global__ __launch_bounds__(256) void k_spmv3_batch(const SpmvJob *__restrict__ jobs)
+{
+    const SpmvJob &jb = jobs[blockIdx.z];
+    const CsrView &m = jb.m[blockIdx.y];
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= m.rows) return;
+    const uint32_t beg = m.row_ptr[row], end = m.row_ptr[row + 1];
+    const fr_t *__restrict__ z = jb.z;
+    fr_t acc = fr_t::zero();
+    for (uint32_t j = beg; j < end; ++j) acc = fp_add(acc, fp_mul(m.values[j], z[m.col_idx[j]]));
+    jb.y[blockIdx.y][row] = acc;
+}
+
 }  // namespace
+
+void spmv3_batch(const SpmvJob *jobs_dev, uint32_t count, uint32_t max_rows, hipStream_t stream)
+{
+    if (count == 0 || max_rows == 0) return;
+    NKR_REQUIRE(count <= 65535, "SpMV batch must hold at most 65535 jobs");
+    k_spmv3_batch<<<dim3(div_up(max_rows, 256), 3, count), 256, 0, stream>>>(jobs_dev);
+    NKR_LAUNCH_CHECK();
+}
 
 void spmv3(const CsrView &a, const CsrView &b, const CsrView &c, const fr_t *z, fr_t *ya, fr_t *yb, fr_t *yc,
            hipStream_t stream)

@@ -106,6 +106,9 @@ def lib() -> ctypes.CDLL:
             "nkr_device_synchronize": [],
             "nkr_g16_verify": [ctypes.POINTER(Vk), vp, u64, vp, ctypes.POINTER(ctypes.c_int)],
             "nkr_g16_setup": [ctypes.POINTER(Circuit), vp, i32, ctypes.POINTER(PkOut), ctypes.POINTER(VkOut)],
+            "nkr_g16_batch_create": [i32, ctypes.POINTER(vp)],
+            "nkr_g16_batch_prove": [vp, ctypes.POINTER(BatchItem), u64],
+            "nkr_g16_batch_destroy": [vp],
             "nkr_bn254_pairing": [vp, vp, vp],
             "nkr_bn254_pairing_product": [vp, vp, u64, vp],
             "nkr_bn254_miller_loop": [vp, vp, vp],
@@ -296,6 +299,48 @@ def setup(circ, trapdoors, device: int = 0) -> Keys:
     vk_out = VkOut(_p(k.alpha_g1_beta_g2), _p(k.gamma_g2), _p(k.delta_g2), _p(k.ic))
     _check(lib().nkr_g16_setup(ctypes.byref(c), _p(td), device, ctypes.byref(pk_out), ctypes.byref(vk_out)))
     return k
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("ctx", ctypes.c_void_p), ("witness", ctypes.c_void_p), ("r", ctypes.c_void_p),
+                ("s", ctypes.c_void_p), ("raw_out", ctypes.c_void_p), ("proof_out", ctypes.c_void_p)]
+
+
+class Batch:
+    """nkr_g16_batch: many proofs (any mix of Prover contexts) as one launch sequence."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib().nkr_g16_batch_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def prove(self, items):
+        """items: (Prover, witness, r, s) tuples -> list of (raw words, proof words)."""
+        keep, outs = [], []
+        arr = (BatchItem * max(len(items), 1))()
+        for i, (pv, w, r, s) in enumerate(items):
+            w = _words(w, 4)
+            assert w.shape[0] == pv.cols
+            r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+            s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+            raw = np.zeros(72, dtype=np.uint64)
+            proof = np.zeros(48, dtype=np.uint64)
+            keep += [w, r, s]
+            outs.append((raw, proof))
+            arr[i] = BatchItem(pv.h.value, _p(w), _p(r), _p(s), _p(raw), _p(proof))
+        _check(lib().nkr_g16_batch_prove(self.h, arr, len(items)))
+        return outs
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().nkr_g16_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Prover:
