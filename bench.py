@@ -7,9 +7,14 @@ domain, 2^20 columns, on BN254.  A step is Backend::prove (SpMV x3, the
 7-transform quotient chain, four G1 MSMs and one G2 MSM) followed by
 blind_proof with r, s from the seed -- the reference's groth16.prove.total
 scope (protocols/groth16/app/prove.cpp:79-88).  The circuit, witness and
-proving key are resident in HBM before the timed region; the proving key is
-synthetic (uniform random multiples of the generators), which has the same
-cost as a trapdoor setup.
+proving key are resident in HBM before the timed region; the keys come from
+the GPU setup with seeded trapdoors, and the timed proof is verified.
+
+Two more legs (skippable): `batch` proves a heterogeneous mix of 64 proofs
+(16 circuit shapes, domains 2^9 .. 2^18) plainly and as one batch, and
+`serving` replays a Poisson request stream (the mix plus 10% 2^20 requests)
+through the plain one-at-a-time path and through continuous batching,
+reporting p50/p99 latency and proofs/s.
 
 Prints one JSON line.  `value` is proofs/s of the whole job; `ms_per_step`
 is the prove latency.  The CPU baseline times the repository's C restatement
@@ -44,44 +49,9 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-class Pk:
-    pass
-
-
-def synthetic_pk(rows: int, cols: int, seed: int) -> Pk:
-    g1, g2 = G1_GENERATOR, G2_GENERATOR
-    pk = Pk()
-    pk.a_query = N.fixed_base_mul(g1, circuits.random_scalars(cols, seed + 1))
-    pk.b_g1_query = N.fixed_base_mul(g1, circuits.random_scalars(cols, seed + 2))
-    pk.b_g2_query = N.fixed_base_mul(g2, circuits.random_scalars(cols, seed + 3), g2=True)
-    lk = circuits.random_scalars(cols, seed + 4)
-    pk.l_query = N.fixed_base_mul(g1, lk)
-    hk = circuits.random_scalars(rows, seed + 5)
-    hk[-1] = 0
-    pk.h_query = N.fixed_base_mul(g1, hk)
-    pk.alpha_g1 = N.fixed_base_mul(g1, circuits.random_scalars(1, seed + 6))[0]
-    pk.beta_g1 = N.fixed_base_mul(g1, circuits.random_scalars(1, seed + 7))[0]
-    pk.delta_g1 = N.fixed_base_mul(g1, circuits.random_scalars(1, seed + 8))[0]
-    pk.beta_g2 = N.fixed_base_mul(g2, circuits.random_scalars(1, seed + 9), g2=True)[0]
-    pk.delta_g2 = N.fixed_base_mul(g2, circuits.random_scalars(1, seed + 10), g2=True)[0]
-    return pk
-
-
-def _point(coords) -> np.ndarray:
-    """Affine integer coordinates (plus Z = 1) -> reference Jacobian words, Montgomery form."""
-    out = np.zeros(4 * len(coords), dtype=np.uint64)
-    for k, v in enumerate(coords):
-        m = v * circuits.MONT % circuits.Q
-        out[4 * k:4 * k + 4] = [(m >> (64 * w)) & 0xFFFFFFFFFFFFFFFF for w in range(4)]
-    return out
-
-
-# alt_bn128 generators (libff alt_bn128_init.cpp:148-150, 207-215)
-G1_GENERATOR = _point([1, 2, 1])
-G2_GENERATOR = _point([10857046999023057135944570762232829481370756359578518086990519993285655852781,
-                       11559732032986387107991004021392285783925812861821192530917403151452391805634,
-                       8495653923123431417604973247489272438418190587263600148770280649306958101930,
-                       4082367875863433681332203403145435568316851327593401208105741076214120093531, 1, 0])
+def percentile(xs, q: float) -> float:
+    s = sorted(xs)
+    return s[min(len(s) - 1, int(round(q / 100.0 * (len(s) - 1))))]
 
 
 def cpu_baseline(circ, pk, witness_np, h_host, budget_s: float) -> dict:
@@ -138,6 +108,10 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mix")
     ap.add_argument("--batch-per-shape", type=int, default=4, help="witnesses per circuit shape in the mix")
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--qps", type=float, default=300.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--requests", type=int, default=300, help="requests in the serving leg")
+    ap.add_argument("--verify-all", action="store_true", help="also verify the served 2^20 proofs")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -149,8 +123,9 @@ def main() -> int:
     log(f"generating matmul {args.size} ({args.density}) seed={SEED:#x}")
     circ = circuits.matmul(args.size, SEED, density=args.density, wide=True)
     log(f"rows={circ.rows} cols={circ.cols} nnz=({circ.a[1].size},{circ.b[1].size},{circ.c[1].size})")
-    pk = synthetic_pk(circ.rows, circ.cols, SEED)
-    log("proving key generated on the GPU")
+    t0 = time.perf_counter()
+    pk = N.setup(circ, circuits.random_scalars(5, SEED ^ 0x5E7))
+    log(f"proving and verification keys generated on the GPU ({time.perf_counter() - t0:.2f}s)")
     t0 = time.perf_counter()
     prover = N.Prover(circ, pk)
     prep_s = time.perf_counter() - t0
@@ -228,7 +203,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32 limbs (BN254 Fr/Fq Montgomery, exact)",
-        "data": "synthetic (seeded matmul R1CS and witness; proving key = random multiples of G1/G2)",
+        "data": "synthetic (seeded matmul R1CS and witness; keys from the GPU setup with seeded trapdoors)",
         "config": {"workload": f"groth16_prove_matmul{args.size}_{args.density}_domain_2^{circ.rows.bit_length() - 1}",
                    "rows": circ.rows, "cols": circ.cols, "path": "plain (one proof per call)",
                    "msm_window_bits": {"witness": cw, "h": ch}},
@@ -248,13 +223,24 @@ def main() -> int:
                      "frac": round(achieved_mul / FQ_MUL_PEAK, 4)},
         },
     }
+    # the timed proof verifies against the verification key from the same setup
+    result["verified"] = bool(N.verify(pk, wit[1:1 + circ.public_inputs], proof))
     h = None
     if not args.no_cpu_baseline:
         _, (_, _, _, h) = prover.prove(wit, intermediates=True)
     N._check(N.lib().nkr_dev_free(dptr))
+    mix_provers = []
+    if not (args.no_batch and args.no_serve):
+        shapes, items = build_mix(args.batch_per_shape)
+        mix_provers = list({id(it[2]): it[2] for it in items}.values())
+        if not args.no_batch:
+            result["batch"] = batch_leg(shapes, items, args.batch_per_shape, args.batch_reps)
+        if not args.no_serve:
+            result["serving"] = serving_leg(items, (prover, pk, circ, wit), args.qps, args.requests,
+                                            args.verify_all)
+    for pv in mix_provers:
+        pv.close()
     prover.close()
-    if not args.no_batch:
-        result["batch"] = batch_leg(args.batch_per_shape, args.batch_reps)
     if not args.no_cpu_baseline:
         log("timing the CPU baseline sample")
         result["cpu_baseline"] = cpu_baseline(circ, pk, wit, h, args.cpu_budget)
@@ -276,28 +262,31 @@ def _msm_window(n: int) -> int:
     return c
 
 
-def batch_leg(per_shape: int, reps: int) -> dict:
+def build_mix(per_shape: int):
+    """The heterogeneous mix as prepared contexts: [(name, circ, prover, keys, witness, r, s)]."""
+    log(f"batch mix: generating 16 circuit shapes x {per_shape} witnesses")
+    t0 = time.perf_counter()
+    mix = circuits.batch_mix(SEED, per_shape)
+    gen_s = time.perf_counter() - t0
+    items = []
+    for k, (name, circ, wits) in enumerate(mix):
+        keys = N.setup(circ, circuits.random_scalars(5, SEED + 17 * k))
+        pv = N.Prover(circ, keys)
+        for j, w in enumerate(wits):
+            rs = circuits.random_scalars(2, SEED ^ (k << 8) ^ j)
+            items.append((name, circ, pv, keys, np.ascontiguousarray(w), rs[0].copy(), rs[1].copy()))
+    log(f"batch mix: {len(items)} proofs ready ({gen_s:.1f}s to generate), rows 2^"
+        f"{min(c.rows for _, c, *_ in items).bit_length() - 1} .. 2^{max(c.rows for _, c, *_ in items).bit_length() - 1}")
+    return [name for name, _, _ in mix], items
+
+
+def batch_leg(shapes, items, per_shape: int, reps: int) -> dict:
     """Batched vs plain proofs/s over the heterogeneous mix (16 circuit shapes,
     domains 2^9 .. 2^18, per_shape witnesses each).  Plain: one
     nkr_g16_prove_blind per proof, back to back.  Batched: the same proofs in
     one nkr_g16_batch_prove call.  Both time host-resident witnesses in and
     blinded proofs out; every batched proof must equal its plain proof bit for
     bit and verify."""
-    log(f"batch mix: generating 16 circuit shapes x {per_shape} witnesses")
-    t0 = time.perf_counter()
-    mix = circuits.batch_mix(SEED, per_shape)
-    gen_s = time.perf_counter() - t0
-    items, keyed = [], []
-    for k, (name, circ, wits) in enumerate(mix):
-        keys = N.setup(circ, circuits.random_scalars(5, SEED + 17 * k))
-        pv = N.Prover(circ, keys)
-        keyed.append((pv, keys))
-        for j, w in enumerate(wits):
-            rs = circuits.random_scalars(2, SEED ^ (k << 8) ^ j)
-            items.append((name, circ, pv, keys, np.ascontiguousarray(w), rs[0].copy(), rs[1].copy()))
-    log(f"batch mix: {len(items)} proofs ready ({gen_s:.1f}s to generate), rows 2^"
-        f"{min(c.rows for _, c, *_ in items).bit_length() - 1} .. 2^{max(c.rows for _, c, *_ in items).bit_length() - 1}")
-
     plain_out = [pv.prove_blind(w, r, s) for (_, _, pv, _, w, r, s) in items]  # warm-up + reference results
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -316,16 +305,71 @@ def batch_leg(per_shape: int, reps: int) -> dict:
     verified = all(N.verify(keys, w[1:1 + circ.public_inputs], o[1])
                    for (_, circ, _, keys, w, _, _), o in zip(items, outs))
     batch.close()
-    for pv, _ in keyed:
-        pv.close()
     n = len(items)
     log(f"batch mix: plain {1000 * plain_s:.1f} ms, batched {1000 * batch_s:.1f} ms for {n} proofs; "
         f"bit-exact={exact} verified={verified}")
-    return {"proofs": n, "shapes": [name for name, _, _ in mix], "per_shape": per_shape,
+    return {"proofs": n, "shapes": shapes, "per_shape": per_shape,
             "plain_ms": round(1000 * plain_s, 3), "batched_ms": round(1000 * batch_s, 3),
             "plain_proofs_per_s": round(n / plain_s, 2), "batched_proofs_per_s": round(n / batch_s, 2),
             "speedup": round(plain_s / batch_s, 3), "bit_exact_vs_plain": exact, "all_verified": verified,
             "reps": reps}
+
+
+def serving_leg(items, big, qps: float, n_req: int, verify_all: bool) -> dict:
+    """A Poisson request stream at `qps` drawn from the mix (90%) and the 2^20
+    matmul context (10%), served twice with the same requests: by the plain
+    one-at-a-time path (1 worker, batches of 1) and by continuous batching
+    (3 workers, batches up to 32, 64 GiB of admitted work memory).  Reports
+    p50/p99 submit-to-completion latency and proofs/s; the batched results
+    must equal the plain ones bit for bit (and verify)."""
+    rng = np.random.default_rng(SEED ^ 0x5E5)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, n_req))
+    picks = []
+    for i in range(n_req):
+        if rng.random() < 0.1:
+            pv, keys, circ, w = big
+            name = "matmul100_sparse"
+        else:
+            name, circ, pv, keys, w, _, _ = items[int(rng.integers(len(items)))]
+        rs = circuits.random_scalars(2, (SEED << 4) ^ i)
+        picks.append((name, circ, pv, keys, w, rs[0].copy(), rs[1].copy()))
+
+    def run(workers: int, max_batch: int):
+        srv = N.Server(workers=workers, max_batch=max_batch, budget_bytes=64 << 30)
+        # warm the workers' buffers on a copy of the stream's largest requests
+        warm = [srv.submit(pv, w, r, s) for (_, _, pv, _, w, r, s) in picks[:max(1, min(8, n_req))]]
+        for t in warm:
+            srv.wait(t)
+        tickets = []
+        t0 = time.perf_counter()
+        for (name, circ, pv, keys, w, r, s), at in zip(picks, arrivals):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(pv, w, r, s))
+        outs = [srv.wait(t) for t in tickets]
+        elapsed = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        lat = [o[2] for o in outs]
+        return outs, {"proofs_per_s": round(n_req / elapsed, 2), "p50_ms": round(percentile(lat, 50), 3),
+                      "p99_ms": round(percentile(lat, 99), 3), "elapsed_s": round(elapsed, 3),
+                      "batches_incl_warmup": int(st["batches"]), "max_batch": int(st["max_batch"])}
+
+    log(f"serving: {n_req} requests at {qps:g}/s (Poisson), plain one-at-a-time")
+    plain_outs, plain = run(1, 1)
+    log(f"serving: plain p50 {plain['p50_ms']} ms p99 {plain['p99_ms']} ms, {plain['proofs_per_s']}/s; "
+        "continuous batching")
+    cb_outs, cb = run(3, 32)
+    log(f"serving: batched p50 {cb['p50_ms']} ms p99 {cb['p99_ms']} ms, {cb['proofs_per_s']}/s")
+    exact = all(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) for a, b in zip(plain_outs, cb_outs))
+    todo = [(p, o) for p, o in zip(picks, cb_outs) if verify_all or p[0] != "matmul100_sparse"]
+    verified = all(N.verify(keys, w[1:1 + circ.public_inputs], o[1]) for (_, circ, _, keys, w, _, _), o in todo)
+    big_share = sum(1 for p in picks if p[0] == "matmul100_sparse")
+    return {"qps": qps, "requests": n_req, "requests_2p20": big_share,
+            "plain": plain, "continuous_batching": cb,
+            "throughput_ratio": round(cb["proofs_per_s"] / plain["proofs_per_s"], 3),
+            "bit_exact_vs_plain": exact, "verified": verified, "verified_count": len(todo)}
 
 
 if __name__ == "__main__":

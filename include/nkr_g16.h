@@ -133,6 +133,30 @@ int nkr_g16_batch_create(int device, nkr_g16_batch **out);
 int nkr_g16_batch_prove(nkr_g16_batch *batch, const nkr_g16_batch_item *items, uint64_t count);
 int nkr_g16_batch_destroy(nkr_g16_batch *batch);
 
+/* Serving: a request loop with continuous batching (no counterpart in the
+ * reference, which serves nothing; its harness runs one prove per process,
+ * scripts/bench/runner.py:260-355).  Contexts stay resident (keys and NTT
+ * tables in HBM); submit() queues a proof and returns a ticket; `workers`
+ * threads each own a batch (own streams), and a free worker takes every
+ * pending request that fits the work-memory budget, up to max_batch, so
+ * requests arriving while other batches are on the GPU are admitted as
+ * workers and memory free up and run alongside them.  Each request is
+ * charged nkr_g16_server_request_bytes of its context; a request larger than
+ * the whole budget is rejected by submit.  wait() blocks until the ticket is
+ * done and gives its submit-to-completion latency.  Results are exactly
+ * nkr_g16_prove_blind's.  The caller keeps witness, r, s and the outputs
+ * alive until wait() returns; submit and wait may be called from any thread.
+ * stats[4] = batches run, proofs done, largest batch, peak admitted bytes. */
+typedef struct nkr_g16_server nkr_g16_server;
+int nkr_g16_server_create(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes,
+                          nkr_g16_server **out);
+int nkr_g16_server_submit(nkr_g16_server *server, nkr_g16_ctx *ctx, const void *witness, const void *r,
+                          const void *s, void *raw_out, void *proof_out, uint64_t *ticket);
+int nkr_g16_server_wait(nkr_g16_server *server, uint64_t ticket, double *latency_ms);
+int nkr_g16_server_request_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes);
+int nkr_g16_server_stats(const nkr_g16_server *server, uint64_t *stats);
+int nkr_g16_server_destroy(nkr_g16_server *server);
+
 /* Stage timing with HIP events on the context's stream.  While enabled,
  * every prove adds its per-stage device time (ms) to running totals:
  *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit
@@ -225,9 +249,13 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream);    /
 
 /* NTT workgroup shape for contexts and domains created afterwards:
  * 2^lds_log elements per workgroup (10 or 11), 2^radix_log elements per
- * thread per register round (2 or 3).  A tuning knob; the default is chosen
- * by measurement. */
-int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log);
+ * thread per register round (2 or 3), and how transforms larger than one
+ * workgroup tile split their bits between passes (0: balanced; 1: the
+ * unit-stride last pass takes a full tile and the strided passes at most
+ * lds_log - 2 bits, so their loads are >= 128-byte contiguous runs).  A
+ * tuning knob: results do not depend on it; the default is chosen by
+ * measurement. */
+int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log, uint32_t split_policy);
 
 /* y = M * z for one CSR matrix (host memory). */
 int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,

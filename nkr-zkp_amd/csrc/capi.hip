@@ -15,6 +15,7 @@
 #include "ntt.h"
 #include "pairing.h"
 #include "prover.h"
+#include "serve.h"
 #include "setup.h"
 #include "spmv.h"
 
@@ -98,6 +99,10 @@ struct nkr_g16_ctx {
 
 struct nkr_g16_batch {
     std::unique_ptr<nkr::BatchProver> prover;
+};
+
+struct nkr_g16_server {
+    std::unique_ptr<nkr::Server> server;
 };
 
 struct nkr_msm {
@@ -262,6 +267,65 @@ int nkr_g16_batch_prove(nkr_g16_batch *batch, const nkr_g16_batch_item *items, u
 int nkr_g16_batch_destroy(nkr_g16_batch *batch)
 {
     return guarded([&] { delete batch; });
+}
+
+int nkr_g16_server_create(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes,
+                          nkr_g16_server **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(out, "null output");
+        *out = nullptr;
+        set_device(device);
+        auto s = std::make_unique<nkr_g16_server>();
+        s->server = std::make_unique<Server>(device, workers, max_batch, work_budget_bytes);
+        *out = s.release();
+    });
+}
+
+int nkr_g16_server_submit(nkr_g16_server *server, nkr_g16_ctx *ctx, const void *witness, const void *r,
+                          const void *s, void *raw_out, void *proof_out, uint64_t *ticket)
+{
+    return guarded([&] {
+        NKR_REQUIRE(server && ctx && r && s && ticket, "null argument");
+        fr_t rr, ss;
+        std::memcpy(&rr, r, sizeof(rr));
+        std::memcpy(&ss, s, sizeof(ss));
+        *ticket = server->server->submit(ctx->prover.get(), witness, rr, ss, raw_out, proof_out);
+    });
+}
+
+int nkr_g16_server_wait(nkr_g16_server *server, uint64_t ticket, double *latency_ms)
+{
+    return guarded([&] {
+        NKR_REQUIRE(server, "null server");
+        const double ms = server->server->wait(ticket);
+        if (latency_ms) *latency_ms = ms;
+    });
+}
+
+int nkr_g16_server_request_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes)
+{
+    return guarded([&] {
+        NKR_REQUIRE(ctx && bytes, "null argument");
+        *bytes = Server::request_bytes(*ctx->prover);
+    });
+}
+
+int nkr_g16_server_stats(const nkr_g16_server *server, uint64_t *stats)
+{
+    return guarded([&] {
+        NKR_REQUIRE(server && stats, "null argument");
+        const Server::Stats s = server->server->stats();
+        stats[0] = s.batches;
+        stats[1] = s.proofs;
+        stats[2] = s.max_batch_seen;
+        stats[3] = s.peak_work_bytes;
+    });
+}
+
+int nkr_g16_server_destroy(nkr_g16_server *server)
+{
+    return guarded([&] { delete server; });
 }
 
 int nkr_g16_profile(nkr_g16_ctx *ctx, int enable)
@@ -437,10 +501,10 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream)
     });
 }
 
-int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log)
+int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log, uint32_t split_policy)
 {
     return guarded([&] {
-        ntt_set_shape(lds_log, radix_log);
+        ntt_set_shape(lds_log, radix_log, split_policy);
         // Cached standalone domains were built with the old shape.
         std::lock_guard<std::mutex> lock(g_ntt_mutex);
         bool any = false;

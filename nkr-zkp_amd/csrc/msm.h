@@ -53,6 +53,7 @@ constexpr uint32_t kMsmPiece = 64;
 constexpr uint32_t kMsmPieceMin = 4;
 constexpr uint32_t kMsmFoldIn = 16;
 constexpr uint32_t kMsmCompleteMax = 256;
+constexpr uint32_t kMaxFoldLevels = 8;
 
 // Window-shifted affine bases of one proving-key vector.
 template <class F>
@@ -111,7 +112,8 @@ private:
     void layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream);
     std::vector<const void *> key_;
     uint32_t fold_levels_ = 0;
-    uint64_t pool_per_job_ = 0, thread_total_ = 0;
+    std::vector<uint64_t> fold_cap_;  // largest partial count of any job at each fold level
+    uint64_t thread_total_ = 0;
     std::vector<uint32_t> thread_base_;
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;

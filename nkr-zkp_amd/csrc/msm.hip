@@ -2,6 +2,7 @@
 #include "msm.h"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <type_traits>
 
@@ -28,6 +29,7 @@ struct JobDev {
     Xyzz<F> *tree[2];              // B each
     Xyzz<F> *planes;               // c
     uint32_t B, c, piece;
+    uint32_t level_base[kMaxFoldLevels];  // where each fold level's partials start in `pool`
 };
 
 // ------------------------------------------------------------ digit recoding
@@ -224,8 +226,8 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
     jb.pool[p] = acc;
 }
 
-// Fold level: buckets with more than kMsmCompleteMax partials get
-// ceil(count / kMsmFoldIn) new partials written at pool[level_base ...].
+// Fold level L: buckets with more than kMsmCompleteMax partials get
+// ceil(count / kMsmFoldIn) new partials written at pool[level_base[L] ...].
 template <class F>
 __global__ __launch_bounds__(1024) void k_fold_scan(const JobDev<F> *__restrict__ jobs)
 {
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(1024) void k_fold_scan(const JobDev<F> *__restrict_
 }
 
 template <class F>
-__global__ __launch_bounds__(256) void k_fold(const JobDev<F> *__restrict__ jobs, uint32_t level_base)
+__global__ __launch_bounds__(256) void k_fold(const JobDev<F> *__restrict__ jobs, uint32_t L)
 {
     const JobDev<F> &jb = jobs[blockIdx.y];
     if (*jb.piece_max <= kMsmCompleteMax) return;
@@ -247,18 +249,18 @@ __global__ __launch_bounds__(256) void k_fold(const JobDev<F> *__restrict__ jobs
     const uint32_t end = min(beg + kMsmFoldIn, jb.reg_start[k] + jb.reg_count[k]);
     Xyzz<F> acc = jb.pool[beg];
     for (uint32_t e = beg + 1; e < end; ++e) acc = xyzz_add(acc, jb.pool[e]);
-    jb.pool[level_base + p] = acc;
+    jb.pool[jb.level_base[L] + p] = acc;
 }
 
 template <class F>
-__global__ __launch_bounds__(256) void k_fold_retarget(const JobDev<F> *__restrict__ jobs, uint32_t level_base)
+__global__ __launch_bounds__(256) void k_fold_retarget(const JobDev<F> *__restrict__ jobs, uint32_t L)
 {
     const JobDev<F> &jb = jobs[blockIdx.y];
     if (*jb.piece_max <= kMsmCompleteMax) return;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= jb.B) return;
     if (jb.reg_count[k] > kMsmCompleteMax) {
-        jb.reg_start[k] = level_base + jb.fold_start[k];
+        jb.reg_start[k] = jb.level_base[L] + jb.fold_start[k];
         jb.reg_count[k] = jb.fold_start[k + 1] - jb.fold_start[k];
     }
 }
@@ -461,33 +463,42 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
     }
     if (key == key_) return;
     const uint32_t J = (uint32_t)jobs.size();
-    uint32_t Bmax = 0;
-    uint64_t pmax = 0;
-    for (const auto &jb : jobs) {
-        const MsmPlan &p = jb.sched->plan();
-        NKR_REQUIRE(jb.bases->plan().n == p.n && jb.bases->plan().c == p.c, "MSM bases and schedule disagree");
-        Bmax = std::max(Bmax, p.B);
-        pmax = std::max<uint64_t>(pmax, jb.sched->max_pieces());
-    }
-    // pool per job: level-0 pieces, then each fold level's worst case
-    uint64_t cap = pmax, total = pmax, cnt = pmax;
+    // Per job, sized by its own plan: the partial-sum pool (level-0 pieces,
+    // then each fold level's worst case) and the per-bucket arrays.
+    std::vector<uint64_t> pool_off(J + 1, 0), b_off(J + 1, 0), b1_off(J + 1, 0);
+    std::vector<std::array<uint32_t, kMaxFoldLevels>> level_base(J);
     fold_levels_ = 0;
-    while (cnt > kMsmCompleteMax) {
-        cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
-        cap = cap / kMsmFoldIn + Bmax + 1;
-        total += cap;
-        ++fold_levels_;
+    fold_cap_.assign(kMaxFoldLevels, 0);
+    for (uint32_t j = 0; j < J; ++j) {
+        const MsmPlan &p = jobs[j].sched->plan();
+        NKR_REQUIRE(jobs[j].bases->plan().n == p.n && jobs[j].bases->plan().c == p.c,
+                    "MSM bases and schedule disagree");
+        uint64_t cap = jobs[j].sched->max_pieces(), cnt = cap, base = cap;
+        uint32_t levels = 0;
+        while (cnt > kMsmCompleteMax) {
+            NKR_REQUIRE(levels < kMaxFoldLevels, "MSM needs too many fold levels");
+            cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
+            cap = cap / kMsmFoldIn + p.B + 1;
+            level_base[j][levels] = (uint32_t)base;
+            fold_cap_[levels] = std::max<uint64_t>(fold_cap_[levels], cap);
+            base += cap;
+            ++levels;
+        }
+        for (uint32_t L = levels; L < kMaxFoldLevels; ++L) level_base[j][L] = (uint32_t)base;
+        NKR_REQUIRE(base < (1ull << 32), "MSM partial pool exceeds 32-bit indexing");
+        fold_levels_ = std::max(fold_levels_, levels);
+        pool_off[j + 1] = pool_off[j] + base;
+        b_off[j + 1] = b_off[j] + p.B;
+        b1_off[j + 1] = b1_off[j] + p.B + 1;
     }
-    NKR_REQUIRE(total < (1ull << 32), "MSM partial pool exceeds 32-bit indexing");
-    pool_per_job_ = total;
     // grow-only: a batch whose composition changes from call to call reuses them
-    pool_.ensure(pool_per_job_ * J);
-    sums_.ensure((size_t)Bmax * J);
-    tree_[0].ensure((size_t)Bmax * J);
-    tree_[1].ensure((size_t)Bmax * J);
-    reg_start_.ensure((size_t)Bmax * J);
-    reg_count_.ensure((size_t)Bmax * J);
-    fold_start_.ensure((size_t)(Bmax + 1) * J);
+    pool_.ensure(pool_off[J]);
+    sums_.ensure(b_off[J]);
+    tree_[0].ensure(b_off[J]);
+    tree_[1].ensure(b_off[J]);
+    reg_start_.ensure(b_off[J]);
+    reg_count_.ensure(b_off[J]);
+    fold_start_.ensure(b1_off[J]);
 
     static_assert(std::is_trivially_copyable<JobDev<F>>::value, "job records are raw-copied");
     JobDev<F> *host = reinterpret_cast<JobDev<F> *>(jobs_stage_.reuse(sizeof(JobDev<F>) * J));
@@ -500,17 +511,18 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.bucket_start = jb.sched->bucket_start_.get();
         d.piece_start = jb.sched->piece_start_.get();
         d.piece_max = jb.sched->bucket_start_.get() + jb.sched->plan().B + 2;
-        d.pool = pool_.get() + pool_per_job_ * j;
-        d.reg_start = reg_start_.get() + (size_t)Bmax * j;
-        d.reg_count = reg_count_.get() + (size_t)Bmax * j;
-        d.fold_start = fold_start_.get() + (size_t)(Bmax + 1) * j;
-        d.sums = sums_.get() + (size_t)Bmax * j;
-        d.tree[0] = tree_[0].get() + (size_t)Bmax * j;
-        d.tree[1] = tree_[1].get() + (size_t)Bmax * j;
+        d.pool = pool_.get() + pool_off[j];
+        d.reg_start = reg_start_.get() + b_off[j];
+        d.reg_count = reg_count_.get() + b_off[j];
+        d.fold_start = fold_start_.get() + b1_off[j];
+        d.sums = sums_.get() + b_off[j];
+        d.tree[0] = tree_[0].get() + b_off[j];
+        d.tree[1] = tree_[1].get() + b_off[j];
         d.planes = jb.planes;
         d.B = jb.sched->plan().B;
         d.c = jb.sched->plan().c;
         d.piece = jb.sched->plan().piece;
+        for (uint32_t L = 0; L < kMaxFoldLevels; ++L) d.level_base[L] = level_base[j][L];
         thread_base_[j + 1] = thread_base_[j] + (uint32_t)jb.sched->max_pieces();
     }
     thread_total_ = thread_base_[J];
@@ -547,19 +559,13 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
         timer->end(kStageAccumulate, stream);
         timer->begin(kStageFold, stream);
     }
-    uint64_t level_base = 0, cap = 0;
-    for (const auto &jb : jobs) cap = std::max<uint64_t>(cap, jb.sched->max_pieces());
-    level_base = cap;
     for (uint32_t L = 0; L < fold_levels_; ++L) {
-        const uint64_t out_cap = cap / kMsmFoldIn + Bmax + 1;
         k_fold_scan<F><<<J, 1024, 0, stream>>>(jd);
         NKR_LAUNCH_CHECK();
-        k_fold<F><<<dim3(div_up(out_cap, 256), J), 256, 0, stream>>>(jd, (uint32_t)level_base);
+        k_fold<F><<<dim3(div_up(fold_cap_[L], 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
-        k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, (uint32_t)level_base);
+        k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
-        level_base += out_cap;
-        cap = out_cap;
     }
     k_complete<F><<<dim3(div_up((uint64_t)Bmax * 16, 256), J), 256, 0, stream>>>(jd);
     NKR_LAUNCH_CHECK();

@@ -82,8 +82,9 @@ private:
 };
 
 // Workgroup shape for domains built afterwards: 2^lds_log LDS elements per
-// workgroup (10 or 11) and 2^radix_log elements per thread per round (2 or 3).
-void ntt_set_shape(unsigned lds_log, unsigned radix_log);
+// workgroup (10 or 11), 2^radix_log elements per thread per round (2 or 3)
+// and the pass split policy (0 balanced, 1 unit-stride last pass full).
+void ntt_set_shape(unsigned lds_log, unsigned radix_log, unsigned split_policy);
 
 // Fr constants shared by host code.
 fr_t fr_from_u64(uint64_t x);

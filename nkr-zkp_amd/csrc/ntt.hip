@@ -17,6 +17,10 @@ constexpr unsigned kMaxThreads = 512;
 // (threads = 2^(lds_log - radix_log)).  Chosen by measurement on MI355X.
 unsigned g_lds_log = 11;
 unsigned g_radix_log = 3;
+// Pass split for transforms larger than one LDS tile: 0 balances the bits
+// between passes, 1 gives the unit-stride last pass a full tile and keeps
+// the strided passes narrow enough to gather several adjacent columns.
+unsigned g_split_policy = 0;
 
 struct PassArgs {
     fr_t *data;
@@ -227,12 +231,14 @@ constexpr unsigned kTwoAdicity = 28;
 
 }  // namespace
 
-void ntt_set_shape(unsigned lds_log, unsigned radix_log)
+void ntt_set_shape(unsigned lds_log, unsigned radix_log, unsigned split_policy)
 {
     NKR_REQUIRE(lds_log == 10 || lds_log == 11, "LDS elements per workgroup must be 2^10 or 2^11");
     NKR_REQUIRE(radix_log == 2 || radix_log == 3, "register radix must be 2^2 or 2^3");
+    NKR_REQUIRE(split_policy <= 1, "pass split policy must be 0 (balanced) or 1 (contiguous last)");
     g_lds_log = lds_log;
     g_radix_log = radix_log;
+    g_split_policy = split_policy;
 }
 
 fr_t fr_from_u64(uint64_t x)
@@ -280,6 +286,14 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
     std::vector<unsigned> split;
     if (log_n <= lds_log_) {
         split = {log_n};
+    } else if (g_split_policy == 1) {
+        // The last pass (unit stride) takes a full LDS tile; the strided passes
+        // before it take at most lds_log - 2 bits, so that each workgroup
+        // gathers >= 4 adjacent columns (>= 128 contiguous bytes per row).
+        const unsigned rest = log_n - lds_log_, cap = lds_log_ - 2;
+        const unsigned k = (rest + cap - 1) / cap;
+        for (unsigned i = 0; i < k; ++i) split.push_back(rest / k + (i < rest % k ? 1 : 0));
+        split.push_back(lds_log_);
     } else if (log_n <= 2 * lds_log_) {
         split = {(log_n + 1) / 2, log_n / 2};
     } else {

@@ -47,6 +47,10 @@ public:
     uint64_t device_bytes() const;
     int device() const { return device_; }
     hipStream_t stream() const { return stream_; }
+    uint32_t rows() const { return rows_; }
+    uint32_t cols() const { return cols_; }
+    const MsmPlan &witness_plan() const { return wplan_; }
+    const MsmPlan &h_plan() const { return hplan_; }
 
 private:
     int device_;

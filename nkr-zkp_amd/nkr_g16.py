@@ -81,7 +81,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_profile": [vp, i32],
             "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
             "nkr_dev_copy": [vp, vp, u64, vp],
-            "nkr_ntt_set_shape": [u32, u32],
+            "nkr_ntt_set_shape": [u32, u32, u32],
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
@@ -109,6 +109,12 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_batch_create": [i32, ctypes.POINTER(vp)],
             "nkr_g16_batch_prove": [vp, ctypes.POINTER(BatchItem), u64],
             "nkr_g16_batch_destroy": [vp],
+            "nkr_g16_server_create": [i32, u32, u32, u64, ctypes.POINTER(vp)],
+            "nkr_g16_server_submit": [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64)],
+            "nkr_g16_server_wait": [vp, u64, ctypes.POINTER(ctypes.c_double)],
+            "nkr_g16_server_request_bytes": [vp, ctypes.POINTER(u64)],
+            "nkr_g16_server_stats": [vp, ctypes.POINTER(u64)],
+            "nkr_g16_server_destroy": [vp],
             "nkr_bn254_pairing": [vp, vp, vp],
             "nkr_bn254_pairing_product": [vp, vp, u64, vp],
             "nkr_bn254_miller_loop": [vp, vp, vp],
@@ -334,6 +340,59 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "h", None):
             lib().nkr_g16_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """nkr_g16_server: continuous-batching request loop over resident contexts."""
+
+    def __init__(self, device: int = 0, workers: int = 2, max_batch: int = 64, budget_bytes: int = 64 << 30):
+        h = ctypes.c_void_p()
+        _check(lib().nkr_g16_server_create(device, workers, max_batch, budget_bytes, ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, pv, witness, r, s) -> int:
+        w = _words(witness, 4)
+        assert w.shape[0] == pv.cols
+        r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+        s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+        raw = np.zeros(72, dtype=np.uint64)
+        proof = np.zeros(48, dtype=np.uint64)
+        t = ctypes.c_uint64()
+        _check(lib().nkr_g16_server_submit(self.h, pv.h, _p(w), _p(r), _p(s), _p(raw), _p(proof), ctypes.byref(t)))
+        self._live[t.value] = (w, r, s, raw, proof)
+        return t.value
+
+    def wait(self, ticket: int):
+        """-> (raw words, proof words, latency ms)"""
+        ms = ctypes.c_double()
+        try:
+            _check(lib().nkr_g16_server_wait(self.h, ticket, ctypes.byref(ms)))
+        finally:
+            w, r, s, raw, proof = self._live.pop(ticket)
+        return raw, proof, ms.value
+
+    def stats(self) -> dict:
+        st = (ctypes.c_uint64 * 4)()
+        _check(lib().nkr_g16_server_stats(self.h, st))
+        return {"batches": st[0], "proofs": st[1], "max_batch": st[2], "peak_work_bytes": st[3]}
+
+    @staticmethod
+    def request_bytes(pv) -> int:
+        b = ctypes.c_uint64()
+        _check(lib().nkr_g16_server_request_bytes(pv.h, ctypes.byref(b)))
+        return b.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            lib().nkr_g16_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,103 @@
+"""Serving (nkr_g16_server_*): requests submitted while others are on the GPU
+are admitted and batched as workers and memory free up; every result equals
+the proof of the same request made alone, and verifies.  Memory admission
+rejects a request larger than the budget and still completes everything
+under a budget that fits one request at a time."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import circuits
+import nkr_g16 as N
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def contexts():
+    out = []
+    specs = [("matmul8", circuits.matmul(8, 1), circuits.matmul(8, 2)),
+             ("fib2048", circuits.fibonacci(2048, 3), circuits.fibonacci(2048, 4)),
+             ("poseidon4", circuits.poseidon(4, 5), circuits.poseidon(4, 6)),
+             ("matmul16s", circuits.matmul(16, 7, density="sparse", wide=True),
+              circuits.matmul(16, 8, density="sparse", wide=True))]
+    for i, (name, c, c2) in enumerate(specs):
+        k = N.setup(c, circuits.random_scalars(5, 300 + i))
+        out.append((name, c, k, N.Prover(c, k), [c.witness, c2.witness]))
+    yield out
+    for _, _, _, pv, _ in out:
+        pv.close()
+
+
+def requests(contexts, n, seed):
+    rs = circuits.random_scalars(2 * n, seed)
+    reqs = []
+    for i in range(n):
+        name, c, k, pv, wits = contexts[(i * 7) % len(contexts)]
+        reqs.append((name, c, k, pv, wits[i % 2], rs[2 * i], rs[2 * i + 1]))
+    return reqs
+
+
+def test_serving_matches_alone_and_batches(contexts):
+    reqs = requests(contexts, 24, 91)
+    alone = [pv.prove_blind(w, r, s) for (_, _, _, pv, w, r, s) in reqs]
+    srv = N.Server(workers=2, max_batch=16)
+    tickets = []
+    # a first burst, then the rest trickling in while the first batches run
+    for q in reqs[:12]:
+        tickets.append(srv.submit(q[3], q[4], q[5], q[6]))
+    for q in reqs[12:]:
+        time.sleep(0.0005)
+        tickets.append(srv.submit(q[3], q[4], q[5], q[6]))
+    got = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    for (name, c, k, _, w, _, _), (raw, proof, ms), (raw1, proof1) in zip(reqs, got, alone):
+        assert np.array_equal(raw, raw1) and np.array_equal(proof, proof1), name
+        assert ms > 0
+        assert N.verify(k, w[1:1 + c.public_inputs], proof), name
+    assert st["proofs"] == len(reqs)
+    assert st["batches"] < len(reqs) and st["max_batch"] >= 2
+
+
+def test_concurrent_submitters(contexts):
+    reqs = requests(contexts, 16, 17)
+    alone = [pv.prove_blind(w, r, s)[1] for (_, _, _, pv, w, r, s) in reqs]
+    srv = N.Server(workers=3, max_batch=8)
+    results = [None] * len(reqs)
+
+    def client(lo, hi):
+        ts = [(i, srv.submit(reqs[i][3], reqs[i][4], reqs[i][5], reqs[i][6])) for i in range(lo, hi)]
+        for i, t in ts:
+            results[i] = srv.wait(t)[1]
+
+    threads = [threading.Thread(target=client, args=(0, 8)), threading.Thread(target=client, args=(8, 16))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    srv.close()
+    for got, want in zip(results, alone):
+        assert np.array_equal(got, want)
+
+
+def test_memory_admission(contexts):
+    name, c, k, pv, wits = contexts[0]
+    need = N.Server.request_bytes(pv)
+    assert need > 0
+    small = N.Server(workers=2, max_batch=8, budget_bytes=need - 1)
+    with pytest.raises(N.NkrError, match="budget"):
+        small.submit(pv, wits[0], *circuits.random_scalars(2, 1))
+    small.close()
+    # room for exactly one request: everything still completes, one at a time
+    one = N.Server(workers=2, max_batch=8, budget_bytes=need)
+    rs = circuits.random_scalars(8, 5)
+    ts = [one.submit(pv, wits[i % 2], rs[2 * i], rs[2 * i + 1]) for i in range(4)]
+    outs = [one.wait(t) for t in ts]
+    st = one.stats()
+    one.close()
+    assert st["max_batch"] == 1 and st["peak_work_bytes"] == need
+    for i, (raw, proof, _) in enumerate(outs):
+        assert np.array_equal(proof, pv.prove_blind(wits[i % 2], rs[2 * i], rs[2 * i + 1])[1])
