@@ -49,6 +49,56 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def pmc_workload(size: int, density: str) -> int:
+    """Workload for `rocprofv3 --pmc` passes (one counter per run): three
+    device-to-device copies of 256 MiB (k_copy16 reads and writes exactly
+    that much, which calibrates FETCH_SIZE / WRITE_SIZE), then two proves of
+    the benchmark circuit with the witness in HBM."""
+    L = N.lib()
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    N._check(L.nkr_dev_alloc(nbytes, ctypes.byref(a)))
+    N._check(L.nkr_dev_alloc(nbytes, ctypes.byref(b)))
+    for _ in range(3):
+        N._check(L.nkr_dev_copy(b, a, nbytes, None))
+    N._check(L.nkr_device_synchronize())
+    log("pmc workload: copies done")
+    circ = circuits.matmul(size, SEED, density=density, wide=True)
+    pk = N.setup(circ, circuits.random_scalars(5, SEED ^ 0x5E7))
+    pv = N.Prover(circ, pk)
+    wit = np.ascontiguousarray(circ.witness)
+    d = ctypes.c_void_p()
+    N._check(L.nkr_dev_alloc(wit.nbytes, ctypes.byref(d)))
+    N._check(L.nkr_memcpy_htod(d, N._p(wit), wit.nbytes))
+    rs = np.ascontiguousarray(circuits.random_scalars(2, SEED ^ 0x72))
+    raw = np.zeros(72, dtype=np.uint64)
+    proof = np.zeros(48, dtype=np.uint64)
+    for _ in range(2):
+        N._check(L.nkr_g16_prove_blind_dev(pv.h, d, N._p(rs[0]), N._p(rs[1]), N._p(raw), N._p(proof)))
+    N._check(L.nkr_device_synchronize())
+    log("pmc workload: proves done")
+    for p in (a, b, d):
+        N._check(L.nkr_dev_free(p))
+    pv.close()
+    return 0
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_prove_matmul100_sparse_2p20.md")
+
+
+def measured_traffic():
+    """HBM bytes per prove of k_accumulate (its three launches), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
+    scaled by the copy-kernel calibration; None if the summary is absent."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith("traffic_bytes_per_prove:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def percentile(xs, q: float) -> float:
     s = sorted(xs)
     return s[min(len(s) - 1, int(round(q / 100.0 * (len(s) - 1))))]
@@ -109,10 +159,17 @@ def main() -> int:
     ap.add_argument("--batch-per-shape", type=int, default=4, help="witnesses per circuit shape in the mix")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    ap.add_argument("--qps", type=float, default=300.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--qps", default="200,2000",
+                    help="Poisson arrival rates of the serving leg (comma list): one below the plain path's "
+                         "capacity for latency, one above both paths' capacity for throughput")
     ap.add_argument("--requests", type=int, default=300, help="requests in the serving leg")
     ap.add_argument("--verify-all", action="store_true", help="also verify the served 2^20 proofs")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="only run the counter-collection workload: three 256 MiB device copies (the "
+                         "calibration kernel), then two proves of the 2^20 circuit")
     args = ap.parse_args()
+    if args.pmc_workload:
+        return pmc_workload(args.size, args.density)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -191,6 +248,10 @@ def main() -> int:
     achieved_gbs = alg_bytes / (acc_ms * 1e-3) / 1e9
     achieved_mul = alg_fq_mul / (acc_ms * 1e-3)
     cw, ch = _msm_window(circ.cols), _msm_window(circ.rows)
+    # measured HBM traffic of the same launches (rocprofv3 counters of this
+    # exact workload), per launch like `achieved`
+    traffic_prove = measured_traffic() if (args.size, args.density) == (100, "sparse") else None
+    traffic = int(traffic_prove / launches) if traffic_prove and launches else None
     result = {
         "metric": "Groth16 prove latency @2^20 constraints; batched proofs/s vs plain (bit-exact)",
         "value": round(1000.0 / ms_per_step, 4),
@@ -214,8 +275,10 @@ def main() -> int:
         "roofline": {
             "kernel": "k_accumulate (MSM bucket accumulation)",
             "bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "launches_per_prove": launches,
             "avg_launch_ms": round(acc_ms / max(1, launches), 4),
+            "alg_bytes_per_launch": int(alg_bytes / max(1, launches)),
             "alg_bytes_per_prove": int(alg_bytes),
             "binding_roof": "integer VALU",
             "valu": {"achieved": round(achieved_mul / 1e9, 2), "peak": round(FQ_MUL_PEAK / 1e9, 1),
@@ -236,8 +299,8 @@ def main() -> int:
         if not args.no_batch:
             result["batch"] = batch_leg(shapes, items, args.batch_per_shape, args.batch_reps)
         if not args.no_serve:
-            result["serving"] = serving_leg(items, (prover, pk, circ, wit), args.qps, args.requests,
-                                            args.verify_all)
+            result["serving"] = [serving_leg(items, (prover, pk, circ, wit), float(q), args.requests,
+                                             args.verify_all) for q in args.qps.split(",")]
     for pv in mix_provers:
         pv.close()
     prover.close()

@@ -160,7 +160,8 @@ int nkr_g16_server_destroy(nkr_g16_server *server);
 /* Stage timing with HIP events on the context's stream.  While enabled,
  * every prove adds its per-stage device time (ms) to running totals:
  *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit
- *   sort (2 schedules), 4 MSM bucket accumulation (5 launches), 5 MSM folding
+ *   sort (2 schedules), 4 MSM bucket accumulation (3 launches: the witness G1
+ *   batch, the G2 MSM, the H MSM), 5 MSM folding
  *   and bit planes, 6 plane download + host finish.
  * counters[0] = proves timed, counters[1] = accumulation launches,
  * counters[2] = bucket entries (non-zero signed digits) of the witness

@@ -14,8 +14,9 @@
 //  * Several MSMs run as one batch ("jobs"): every stage below is a single
 //    launch over all jobs, so the latency of each reduction level is paid
 //    once per batch, not once per MSM.
-//      1. accumulate: one thread sums up to 64 entries of one bucket
-//         (mixed affine additions into XYZZ);
+//      1. accumulate: one thread sums a piece of 4..64 entries of one bucket
+//         (mixed affine additions into XYZZ); the launch holds only the
+//         resident waves, which take pieces by grid stride;
 //      2. fold: buckets left with more than 256 partial sums (skewed digit
 //         distributions, e.g. witnesses that are mostly 0 or 1) are folded
 //         16:1 until they are not; other buckets are untouched;
@@ -118,7 +119,6 @@ private:
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;
     PinnedStaging<uint8_t> jobs_stage_;
-    PinnedStaging<uint32_t> base_stage_;
     DeviceBuffer<Xyzz<F>> pool_, sums_, tree_[2];
     DeviceBuffer<uint32_t> reg_start_, reg_count_, fold_start_;
 };

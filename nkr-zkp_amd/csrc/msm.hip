@@ -198,32 +198,77 @@ __global__ __launch_bounds__(256) void k_regions_init(const JobDev<F> *__restric
     jb.reg_count[k] = jb.piece_start[k + 1] - jb.piece_start[k];
 }
 
-// Level 0 over all jobs: one thread sums up to `piece` signed bases of one bucket.
+// piece_base[j] = level-0 pieces of jobs before j (as scheduled on the
+// device), piece_base[J] = all of them.  One workgroup.
+template <class F>
+__global__ __launch_bounds__(1024) void k_piece_scan(const JobDev<F> *__restrict__ jobs, uint32_t J,
+                                                     uint32_t *__restrict__ piece_base)
+{
+    __shared__ uint32_t part[1024];
+    const uint32_t per = (J + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = min(J, threadIdx.x * per), end = min(J, beg + per);
+    uint32_t sum = 0;
+    for (uint32_t j = beg; j < end; ++j) sum += jobs[j].piece_start[jobs[j].B];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t j = beg; j < end; ++j) {
+        piece_base[j] = run;
+        run += jobs[j].piece_start[jobs[j].B];
+    }
+    if (threadIdx.x == blockDim.x - 1) piece_base[J] = part[threadIdx.x];
+}
+
+// Level 0 over all jobs: a thread sums `piece` signed bases of one bucket
+// into one partial, then moves on by the grid stride, so the launch can be
+// sized to the resident waves and every wave gets the same share of pieces.
 template <class F>
 __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
-                                                    const uint32_t *__restrict__ thread_base, uint32_t J)
+                                                    const uint32_t *__restrict__ piece_base, uint32_t J)
 {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= thread_base[J]) return;
-    const uint32_t j = find_segment(thread_base, J, g);
-    const JobDev<F> &jb = jobs[j];
-    const uint32_t p = g - thread_base[j];
-    if (p >= jb.piece_start[jb.B]) return;
-    const uint32_t k = find_segment(jb.piece_start, jb.B, p);
-    const uint32_t q = p - jb.piece_start[k];
-    const uint32_t beg = jb.bucket_start[k] + q * jb.piece;
-    const uint32_t end = min(beg + jb.piece, jb.bucket_start[k + 1]);
-    const Affine<F> *__restrict__ bases = jb.bases;
-    const uint32_t *__restrict__ sorted = jb.sorted;
-    Xyzz<F> acc = Xyzz<F>::infinity();
-    for (uint32_t e = beg; e < end; ++e) {
-        const uint32_t v = sorted[e];
-        const Affine<F> P = bases[v & kEntryMask];
-        if (affine_is_inf(P)) continue;
-        const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
-        acc = xyzz_add_affine(acc, P.x, y);
+    const uint32_t total = piece_base[J];
+#pragma unroll 1
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        const uint32_t j = find_segment(piece_base, J, g);
+        const JobDev<F> &jb = jobs[j];
+        const uint32_t p = g - piece_base[j];
+        const uint32_t k = find_segment(jb.piece_start, jb.B, p);
+        const uint32_t q = p - jb.piece_start[k];
+        const uint32_t beg = jb.bucket_start[k] + q * jb.piece;
+        const uint32_t end = min(beg + jb.piece, jb.bucket_start[k + 1]);
+        const Affine<F> *__restrict__ bases = jb.bases;
+        const uint32_t *__restrict__ sorted = jb.sorted;
+        Xyzz<F> acc = Xyzz<F>::infinity();
+        if (beg < end) {
+            // Software pipeline: the gather of entry e + 1's base is issued
+            // before entry e's addition, so its latency hides behind the
+            // field arithmetic.
+            uint32_t v = sorted[beg];
+            Affine<F> P = bases[v & kEntryMask];
+#pragma unroll 1
+            for (uint32_t e = beg; e < end; ++e) {
+                uint32_t vn = 0;
+                Affine<F> Pn;
+                if (e + 1 < end) {
+                    vn = sorted[e + 1];
+                    Pn = bases[vn & kEntryMask];
+                }
+                if (!affine_is_inf(P)) {
+                    const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
+                    acc = xyzz_add_affine(acc, P.x, y);
+                }
+                v = vn;
+                P = Pn;
+            }
+        }
+        jb.pool[p] = acc;
     }
-    jb.pool[p] = acc;
 }
 
 // Fold level L: buckets with more than kMsmCompleteMax partials get
@@ -378,6 +423,21 @@ __global__ __launch_bounds__(128) void k_prepare_bases(const Jacobian<F> *__rest
     }
 }
 
+// Workgroups of k_accumulate<F> the device keeps resident at once (all CUs at
+// the kernel's occupancy), times two so that the grid-stride loop ends evenly.
+template <class F>
+uint64_t resident_accumulate_blocks()
+{
+    static const uint64_t blocks = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        NKR_HIP(hipGetDevice(&dev));
+        NKR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        NKR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_accumulate<F>, 256, 0));
+        return (uint64_t)std::max(1, cus) * std::max(1, per_cu) * 2;
+    }();
+    return blocks;
+}
+
 }  // namespace
 
 MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
@@ -405,10 +465,11 @@ MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.B = 1u << (c - 1);
-    // about 2^17 accumulation threads when the entries allow it
-    const uint64_t want = p.max_entries() >> 17;
+    // pieces of up to n*W / 2^19 entries: several per resident thread, so the
+    // grid-stride accumulation stays balanced, and short chains for small MSMs
+    const uint64_t want = p.max_entries() >> 19;
     p.piece = kMsmPieceMin;
-    while (p.piece < kMsmPiece && p.piece < want) p.piece <<= 1;
+    while (p.piece < kMsmPiece && 2 * p.piece <= want) p.piece <<= 1;
     NKR_REQUIRE((uint64_t)p.W * n < (1ull << 31), "MSM too large for 31-bit entry indices");
     return p;
 }
@@ -526,14 +587,10 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         thread_base_[j + 1] = thread_base_[j] + (uint32_t)jb.sched->max_pieces();
     }
     thread_total_ = thread_base_[J];
-    uint32_t *tb = base_stage_.reuse(J + 1);
-    std::memcpy(tb, thread_base_.data(), (J + 1) * sizeof(uint32_t));
     jobs_dev_.ensure(sizeof(JobDev<F>) * J);
-    thread_base_dev_.ensure(J + 1);
+    thread_base_dev_.ensure(J + 1);  // actual piece offsets, filled on the device per run
     NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host, sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
     jobs_stage_.uploaded(stream);
-    NKR_HIP(hipMemcpyAsync(thread_base_dev_.get(), tb, (J + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
-    base_stage_.uploaded(stream);
     key_ = key;
 }
 
@@ -553,7 +610,10 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     k_regions_init<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd);
     NKR_LAUNCH_CHECK();
     if (timer) timer->begin(kStageAccumulate, stream);
-    k_accumulate<F><<<div_up(thread_total_, 256), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
+    k_piece_scan<F><<<1, 1024, 0, stream>>>(jd, J, thread_base_dev_.get());
+    NKR_LAUNCH_CHECK();
+    const uint64_t blocks = std::min<uint64_t>(div_up(thread_total_, 256), resident_accumulate_blocks<F>());
+    k_accumulate<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
     NKR_LAUNCH_CHECK();
     if (timer) {
         timer->end(kStageAccumulate, stream);
