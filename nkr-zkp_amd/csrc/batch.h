@@ -7,11 +7,11 @@
 //   SpMV        one launch over (job, matrix, row)
 //   NTT chain   one launch per pass per size class (all A, B, C of the class)
 //   quotient    one launch per size class
-//   sort        per schedule (witness and H of every proof)
-//   MSM         one MsmBatch for all witness G1 MSMs (3 per proof) and the
-//               G2 MSMs on a side stream, one for all H MSMs on the main
-//               stream: each accumulation / fold / tree level is one launch
-//               for the whole batch
+//   sort        one launch per sort phase for all witness and H schedules
+//   MSM         one MsmBatch for all G1 MSMs (4 per proof) and one for the
+//               G2 MSMs: each accumulation / fold / completion / tree level is
+//               one launch for the whole batch
+// All on one stream: each stage already fills the device.
 //   host tail   the r,s-only blinding terms while the GPU works, then the
 //               final sums and blinding on a pool of host threads.
 // Each proof's result is bit-identical to proving it alone.
@@ -51,10 +51,10 @@ private:
         DeviceBuffer<Xyzz<fq2_t>> planes2;
     };
     int device_;
-    hipStream_t stream_ = nullptr, stream2_ = nullptr;
-    hipEvent_t ev_start_ = nullptr, ev_side_done_ = nullptr;
+    hipStream_t stream_ = nullptr;
     std::vector<std::unique_ptr<Slot>> slots_;
-    MsmBatch<fq_t> batch_w_, batch_h_;
+    MsmScheduleBatch sort_batch_;
+    MsmBatch<fq_t> batch_g1_;
     MsmBatch<fq2_t> batch_g2_;
     PinnedStaging<uint8_t> witness_stage_, table_stage_;
     DeviceBuffer<uint8_t> tables_;

@@ -41,29 +41,23 @@ void parallel_for(size_t n, Fn fn)
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+// Accumulation piece length for batched MSMs.
+constexpr uint32_t kBatchPiece = 32;
+
 }  // namespace
 
 BatchProver::BatchProver(int device) : device_(device)
 {
     NKR_HIP(hipSetDevice(device_));
     NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
-    NKR_HIP(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
-    NKR_HIP(hipEventCreateWithFlags(&ev_side_done_, hipEventDisableTiming));
 }
 
 BatchProver::~BatchProver()
 {
-    if (stream2_) {
-        (void)hipStreamSynchronize(stream2_);
-        (void)hipStreamDestroy(stream2_);
-    }
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
     }
-    if (ev_start_) (void)hipEventDestroy(ev_start_);
-    if (ev_side_done_) (void)hipEventDestroy(ev_side_done_);
 }
 
 void BatchProver::prove(const std::vector<BatchItem> &items)
@@ -89,20 +83,21 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         max_rows = std::max(max_rows, g.rows_);
     }
     uint8_t *wst = witness_stage_.reuse(wbytes);
+    std::vector<size_t> woff(K + 1, 0);
+    for (size_t i = 0; i < K; ++i) woff[i + 1] = woff[i] + (size_t)items[i].ctx->cols_ * sizeof(fr_t);
+    // host copies into the pinned staging buffer in parallel (host memory bandwidth)
+    parallel_for(K, [&](size_t i) { std::memcpy(wst + woff[i], items[i].witness, woff[i + 1] - woff[i]); });
     size_t off = 0;
     for (size_t i = 0; i < K; ++i) {
         const Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
-        const size_t bytes = (size_t)g.cols_ * sizeof(fr_t);
-        std::memcpy(wst + off, items[i].witness, bytes);
         sl.z.ensure(g.cols_);
         sl.pa.ensure(g.rows_);
         sl.pb.ensure(g.rows_);
         sl.pc.ensure(g.rows_);
         sl.planes1.ensure(3 * g.wplan_.c + g.hplan_.c);
         sl.planes2.ensure(g.wplan_.c);
-        NKR_HIP(hipMemcpyAsync(sl.z.get(), wst + off, bytes, hipMemcpyHostToDevice, stream_));
-        off += bytes;
+        NKR_HIP(hipMemcpyAsync(sl.z.get(), wst + woff[i], woff[i + 1] - woff[i], hipMemcpyHostToDevice, stream_));
     }
     witness_stage_.uploaded(stream_);
 
@@ -151,27 +146,11 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     }
     NKR_HIP(hipMemcpyAsync(tables_.get(), tst, tbytes, hipMemcpyHostToDevice, stream_));
     table_stage_.uploaded(stream_);
-    NKR_HIP(hipEventRecord(ev_start_, stream_));
 
-    // side stream: witness schedules, then every witness MSM of the batch
-    NKR_HIP(hipStreamWaitEvent(stream2_, ev_start_, 0));
-    std::vector<MsmJob<fq_t>> wjobs;
-    std::vector<MsmJob<fq2_t>> g2jobs;
-    for (size_t i = 0; i < K; ++i) {
-        Groth16Prover &g = *items[i].ctx;
-        Slot &sl = *slots_[i];
-        sl.wsched.build(sl.z.get(), g.wplan_, stream2_);
-        const uint32_t cw = g.wplan_.c;
-        wjobs.push_back({&g.a_, &sl.wsched, sl.planes1.get() + 0 * cw});
-        wjobs.push_back({&g.b1_, &sl.wsched, sl.planes1.get() + 1 * cw});
-        wjobs.push_back({&g.l_, &sl.wsched, sl.planes1.get() + 2 * cw});
-        g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
-    }
-    batch_w_.run(wjobs, stream2_);
-    batch_g2_.run(g2jobs, stream2_);
-    NKR_HIP(hipEventRecord(ev_side_done_, stream2_));
-
-    // main stream: A*z, B*z, C*z, the quotient chain per size class, H schedules and MSMs
+    // One stream: every stage is already one launch over the whole batch and
+    // fills the device, so a second stream would only make the stages wait
+    // for each other's workgroup slots.
+    // A*z, B*z, C*z, then the quotient chain per size class
     spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, stream_);
     for (const ClassTables &ct : ctabs) {
         fr_t *const *all3 = reinterpret_cast<fr_t *const *>(tables_.get() + ct.all3);
@@ -182,15 +161,36 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         ct.domain->quotient_batch(quot, ct.count, stream_);
         ct.domain->dif_batch(a_only, ct.count, true, nullptr, ct.domain->icoset_rev_scaled(), nullptr, stream_);
     }
-    std::vector<MsmJob<fq_t>> hjobs;
+    // digit schedules of every witness and every H, then one G1 batch over
+    // all four G1 MSMs of every proof and one G2 batch
+    std::vector<MsmJob<fq_t>> g1jobs;
+    std::vector<MsmJob<fq2_t>> g2jobs;
+    std::vector<MsmScheduleBatch::Item> sorts;
     for (size_t i = 0; i < K; ++i) {
         Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
-        sl.hsched.build(sl.pa.get(), g.hplan_, stream_);
-        hjobs.push_back({&g.h_, &sl.hsched, sl.planes1.get() + 3 * g.wplan_.c});
+        // In a batch the many jobs already fill the device, so pieces are
+        // long (fewer partial sums to complete); one proof alone uses short
+        // pieces for parallelism.  Results do not depend on the piece length.
+        MsmPlan wp = g.wplan_, hp = g.hplan_;
+        wp.piece = std::max(wp.piece, kBatchPiece);
+        hp.piece = std::max(hp.piece, kBatchPiece);
+        sorts.push_back({&sl.wsched, sl.z.get(), wp});
+        sorts.push_back({&sl.hsched, sl.pa.get(), hp});
     }
-    batch_h_.run(hjobs, stream_);
-    NKR_HIP(hipStreamWaitEvent(stream_, ev_side_done_, 0));
+    sort_batch_.build(sorts, stream_);
+    for (size_t i = 0; i < K; ++i) {
+        Groth16Prover &g = *items[i].ctx;
+        Slot &sl = *slots_[i];
+        const uint32_t cw = g.wplan_.c;
+        g1jobs.push_back({&g.a_, &sl.wsched, sl.planes1.get() + 0 * cw});
+        g1jobs.push_back({&g.b1_, &sl.wsched, sl.planes1.get() + 1 * cw});
+        g1jobs.push_back({&g.l_, &sl.wsched, sl.planes1.get() + 2 * cw});
+        g1jobs.push_back({&g.h_, &sl.hsched, sl.planes1.get() + 3 * cw});
+        g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
+    }
+    batch_g1_.run(g1jobs, stream_);
+    batch_g2_.run(g2jobs, stream_);
 
     // bit planes of every proof to the host
     uint8_t *ph = planes_host_.reuse(pbytes);

@@ -79,6 +79,8 @@ private:
 class MsmSchedule {
 public:
     void build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer = nullptr);
+    // Sizes the buffers for `plan`; returns the scalars per sort workgroup.
+    uint32_t prepare(const MsmPlan &plan);
     const MsmPlan &plan() const { return plan_; }
     uint64_t max_pieces() const { return plan_.max_entries() / plan_.piece + plan_.B + 1; }
     // Device counters: [0] entries (non-zero digits), [1] level-0 pieces.
@@ -91,6 +93,23 @@ public:
     DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the piece total at [B + 1]
     DeviceBuffer<uint32_t> piece_start_;   // B + 1
     DeviceBuffer<uint32_t> block_base_;    // blocks * B
+};
+
+// Many schedules with one launch per sort phase (zero, histogram, scan,
+// scatter) instead of four launches per schedule.  Same result as
+// MsmSchedule::build for each of them.
+class MsmScheduleBatch {
+public:
+    struct Item {
+        MsmSchedule *sched;
+        const fr_t *scalars_mont;
+        MsmPlan plan;
+    };
+    void build(const std::vector<Item> &items, hipStream_t stream);
+
+private:
+    PinnedStaging<uint8_t> stage_;
+    DeviceBuffer<uint8_t> table_;
 };
 
 // Stage indices used with StageTimer by the MSM code.
@@ -115,7 +134,9 @@ private:
     uint32_t fold_levels_ = 0;
     std::vector<uint64_t> fold_cap_;  // largest partial count of any job at each fold level
     uint64_t thread_total_ = 0;
-    std::vector<uint32_t> thread_base_;
+    uint64_t complete_bound_ = 0;                      // completion threads at 16 lanes per bucket
+    DeviceBuffer<uint32_t> complete_base_, complete_lanes_;
+    std::vector<uint64_t> thread_base_;
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;
     PinnedStaging<uint8_t> jobs_stage_;

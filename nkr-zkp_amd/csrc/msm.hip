@@ -74,14 +74,13 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n)
 
 // Block-local histogram of bucket magnitudes, then one global atomic per
 // (block, bucket) reserves the block's range inside each bucket.
-__global__ __launch_bounds__(256) void k_hist(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                                              uint32_t B, uint32_t per_block, uint32_t *__restrict__ count,
-                                              uint32_t *__restrict__ block_base)
+__device__ __forceinline__ void hist_block(uint32_t *hist, const fr_t *__restrict__ scalars, uint32_t n, uint32_t c,
+                                          uint32_t W, uint32_t B, uint32_t per_block, uint32_t b,
+                                          uint32_t *__restrict__ count, uint32_t *__restrict__ block_base)
 {
-    extern __shared__ uint32_t hist[];
     for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) hist[k] = 0;
     __syncthreads();
-    const uint32_t beg = blockIdx.x * per_block;
+    const uint32_t beg = b * per_block;
     const uint32_t end = min(n, beg + per_block);
     for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
         const fr_t s = fp_from_mont(scalars[i]);
@@ -91,23 +90,22 @@ __global__ __launch_bounds__(256) void k_hist(const fr_t *__restrict__ scalars, 
         });
     }
     __syncthreads();
-    uint32_t *mine = block_base + (size_t)blockIdx.x * B;
+    uint32_t *mine = block_base + (size_t)b * B;
     for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) {
         const uint32_t h = hist[k];
         mine[k] = h ? atomicAdd(&count[k], h) : 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                                                 uint32_t B, uint32_t per_block,
-                                                 const uint32_t *__restrict__ bucket_start,
-                                                 const uint32_t *__restrict__ block_base, uint32_t *__restrict__ sorted)
+__device__ __forceinline__ void scatter_block(uint32_t *cursor, const fr_t *__restrict__ scalars, uint32_t n,
+                                             uint32_t c, uint32_t W, uint32_t B, uint32_t per_block, uint32_t b,
+                                             const uint32_t *__restrict__ bucket_start,
+                                             const uint32_t *__restrict__ block_base, uint32_t *__restrict__ sorted)
 {
-    extern __shared__ uint32_t cursor[];
-    const uint32_t *mine = block_base + (size_t)blockIdx.x * B;
+    const uint32_t *mine = block_base + (size_t)b * B;
     for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) cursor[k] = bucket_start[k] + mine[k];
     __syncthreads();
-    const uint32_t beg = blockIdx.x * per_block;
+    const uint32_t beg = b * per_block;
     const uint32_t end = min(n, beg + per_block);
     for (uint32_t i = beg + threadIdx.x; i < end; i += blockDim.x) {
         const fr_t s = fp_from_mont(scalars[i]);
@@ -118,6 +116,24 @@ __global__ __launch_bounds__(256) void k_scatter(const fr_t *__restrict__ scalar
         });
     }
 }
+
+__global__ __launch_bounds__(256) void k_hist(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
+                                              uint32_t B, uint32_t per_block, uint32_t *__restrict__ count,
+                                              uint32_t *__restrict__ block_base)
+{
+    extern __shared__ uint32_t lds[];
+    hist_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, count, block_base);
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
+                                                 uint32_t B, uint32_t per_block,
+                                                 const uint32_t *__restrict__ bucket_start,
+                                                 const uint32_t *__restrict__ block_base, uint32_t *__restrict__ sorted)
+{
+    extern __shared__ uint32_t lds[];
+    scatter_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, bucket_start, block_base, sorted);
+}
+
 
 // Exclusive prefix of `len` values into out[0..len] within one workgroup.
 //   mode 0: v[k] = src[k]
@@ -186,6 +202,49 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t *starts, uint32_
             hi = mid;
     }
     return lo;
+}
+
+// One schedule of a batched sort.
+struct SortJob {
+    const fr_t *scalars;
+    uint32_t n, c, W, B, per_block, piece;
+    uint32_t *count, *block_base, *bucket_start, *piece_start, *sorted;
+};
+
+__global__ __launch_bounds__(256) void k_zero_counts(const SortJob *__restrict__ jobs)
+{
+    const SortJob &jb = jobs[blockIdx.y];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < jb.B) jb.count[k] = 0;
+}
+
+// Workgroup blockIdx.x is sort block (blockIdx.x - block_off[j]) of job j.
+__global__ __launch_bounds__(256) void k_hist_batch(const SortJob *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ block_off, uint32_t J)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t j = find_segment(block_off, J, blockIdx.x);
+    const SortJob &jb = jobs[j];
+    hist_block(lds, jb.scalars, jb.n, jb.c, jb.W, jb.B, jb.per_block, blockIdx.x - block_off[j], jb.count,
+               jb.block_base);
+}
+
+__global__ __launch_bounds__(1024) void k_schedule_scan_batch(const SortJob *__restrict__ jobs)
+{
+    const SortJob &jb = jobs[blockIdx.x];
+    block_scan(jb.count, jb.B, 0, 1, 0, jb.bucket_start);
+    block_scan(jb.count, jb.B, 2, jb.piece, 0, jb.piece_start, jb.bucket_start + jb.B + 2);
+    if (threadIdx.x == 0) jb.bucket_start[jb.B + 1] = jb.piece_start[jb.B];
+}
+
+__global__ __launch_bounds__(256) void k_scatter_batch(const SortJob *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ block_off, uint32_t J)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t j = find_segment(block_off, J, blockIdx.x);
+    const SortJob &jb = jobs[j];
+    scatter_block(lds, jb.scalars, jb.n, jb.c, jb.W, jb.B, jb.per_block, blockIdx.x - block_off[j],
+                  jb.bucket_start, jb.block_base, jb.sorted);
 }
 
 template <class F>
@@ -321,23 +380,70 @@ __device__ __forceinline__ Xyzz<F> shfl_xor_point(const Xyzz<F> &p, int mask)
     return r;
 }
 
-// Bucket completion: 16 lanes per bucket (<= kMsmCompleteMax partials).
+// Lanes per bucket for each job's completion and the job offsets of the
+// completion index space.  A bucket with m partials summed by L lanes costs
+// ceil(m / L) - 1 serial additions plus log2 L shuffle levels; L =
+// nextpow2(m) / 2 (1 .. 16) minimises that for the job's largest bucket
+// (m <= kMsmCompleteMax after folding).  Jobs are padded to whole waves so
+// that every wave completes buckets of one job with one L.
 template <class F>
-__global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ jobs)
+__global__ __launch_bounds__(1024) void k_complete_scan(const JobDev<F> *__restrict__ jobs, uint32_t J,
+                                                        uint32_t *__restrict__ base, uint32_t *__restrict__ lanes)
 {
-    const JobDev<F> &jb = jobs[blockIdx.y];
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t k = gid >> 4;
-    const uint32_t lane = gid & 15u;
-    Xyzz<F> acc = Xyzz<F>::infinity();
-    if (k < jb.B) {
-        const uint32_t base = jb.reg_start[k];
-        const uint32_t cnt = jb.reg_count[k];
-        for (uint32_t i = lane; i < cnt; i += 16) acc = xyzz_add(acc, jb.pool[base + i]);
+    __shared__ uint32_t part[1024];
+    auto job_lanes = [&](uint32_t j) -> uint32_t {
+        const uint32_t m = min(*jobs[j].piece_max, kMsmCompleteMax);
+        uint32_t L = 1;
+        while (2 * L < m && L < 16) L <<= 1;  // nextpow2(m) / 2
+        return L;
+    };
+    auto job_span = [&](uint32_t j, uint32_t L) -> uint32_t { return (jobs[j].B * L + 63u) & ~63u; };
+    const uint32_t per = (J + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = min(J, threadIdx.x * per), end = min(J, beg + per);
+    uint32_t sum = 0;
+    for (uint32_t j = beg; j < end; ++j) sum += job_span(j, job_lanes(j));
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
     }
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, off));
-    if (k < jb.B && lane == 0) jb.sums[k] = acc;
+    uint32_t run = part[threadIdx.x] - sum;
+    for (uint32_t j = beg; j < end; ++j) {
+        const uint32_t L = job_lanes(j);
+        base[j] = run;
+        lanes[j] = L;
+        run += job_span(j, L);
+    }
+    if (threadIdx.x == blockDim.x - 1) base[J] = part[threadIdx.x];
+}
+
+// Bucket completion: L lanes per bucket stride over its partials, then a
+// shuffle tree of log2 L levels; grid-stride over all jobs' buckets.
+template <class F>
+__global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ jobs,
+                                                  const uint32_t *__restrict__ base,
+                                                  const uint32_t *__restrict__ lanes, uint32_t J)
+{
+    const uint32_t total = base[J];
+#pragma unroll 1
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
+        const uint32_t j = find_segment(base, J, g);
+        const JobDev<F> &jb = jobs[j];
+        const uint32_t L = lanes[j];
+        const uint32_t local = g - base[j];
+        const uint32_t k = local / L, lane = local & (L - 1);
+        Xyzz<F> acc = Xyzz<F>::infinity();
+        if (k < jb.B) {
+            const uint32_t start = jb.reg_start[k];
+            const uint32_t cnt = jb.reg_count[k];
+            for (uint32_t i = lane; i < cnt; i += L) acc = xyzz_add(acc, jb.pool[start + i]);
+        }
+        for (uint32_t off = L >> 1; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, (int)off));
+        if (k < jb.B && lane == 0) jb.sums[k] = acc;
+    }
 }
 
 // Bit-plane tree, level t -> t+1.  A level-t block covers buckets
@@ -423,16 +529,17 @@ __global__ __launch_bounds__(128) void k_prepare_bases(const Jacobian<F> *__rest
     }
 }
 
-// Workgroups of k_accumulate<F> the device keeps resident at once (all CUs at
-// the kernel's occupancy), times two so that the grid-stride loop ends evenly.
-template <class F>
-uint64_t resident_accumulate_blocks()
+// Workgroups of 256 threads of `kernel` the device keeps resident at once
+// (all CUs at the kernel's occupancy), times two so that grid-stride loops
+// end evenly.  Computed once per kernel.
+template <auto Kernel>
+uint64_t resident_blocks()
 {
     static const uint64_t blocks = [] {
         int dev = 0, cus = 0, per_cu = 0;
         NKR_HIP(hipGetDevice(&dev));
         NKR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        NKR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_accumulate<F>, 256, 0));
+        NKR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, 256, 0));
         return (uint64_t)std::max(1, cus) * std::max(1, per_cu) * 2;
     }();
     return blocks;
@@ -485,17 +592,23 @@ void MsmBases<F>::prepare(const Jacobian<F> *jacobian, const uint32_t *perm, con
     NKR_LAUNCH_CHECK();
 }
 
-void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer)
+uint32_t MsmSchedule::prepare(const MsmPlan &plan)
 {
     plan_ = plan;
     const uint32_t n = plan.n, B = plan.B;
     blocks_ = std::min<uint32_t>(kSortBlocksMax, div_up(n, 256));
-    const uint32_t per_block = div_up(n, blocks_);
     sorted_.ensure((size_t)n * plan.W);
     count_.ensure(B);
     bucket_start_.ensure(B + 3);  // + total pieces, + largest piece count of a bucket
     piece_start_.ensure(B + 1);
     block_base_.ensure((size_t)blocks_ * B);
+    return div_up(n, blocks_);
+}
+
+void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream_t stream, StageTimer *timer)
+{
+    const uint32_t per_block = prepare(plan);
+    const uint32_t n = plan.n, B = plan.B;
     const size_t lds = (size_t)B * sizeof(uint32_t);
     if (timer) timer->begin(kStageSort, stream);
     k_zero_u32<<<div_up(B, 256), 256, 0, stream>>>(count_.get(), B);
@@ -509,6 +622,47 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
                                              block_base_.get(), sorted_.get());
     NKR_LAUNCH_CHECK();
     if (timer) timer->end(kStageSort, stream);
+}
+
+void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
+{
+    const uint32_t J = (uint32_t)items.size();
+    if (J == 0) return;
+    NKR_REQUIRE(J <= 65535, "a schedule batch holds at most 65535 schedules");
+    const size_t jobs_bytes = (sizeof(SortJob) * J + 15) & ~size_t(15);
+    const size_t total = jobs_bytes + sizeof(uint32_t) * (J + 1);
+    uint8_t *st = stage_.reuse(total);
+    table_.ensure(total);
+    SortJob *jobs = reinterpret_cast<SortJob *>(st);
+    uint32_t *off = reinterpret_cast<uint32_t *>(st + jobs_bytes);
+    uint32_t Bmax = 0;
+    uint64_t blocks = 0;
+    for (uint32_t j = 0; j < J; ++j) {
+        MsmSchedule &s = *items[j].sched;
+        const MsmPlan &p = items[j].plan;
+        const uint32_t per_block = s.prepare(p);
+        jobs[j] = SortJob{items[j].scalars_mont, p.n, p.c, p.W, p.B, per_block, p.piece,
+                          s.count_.get(), s.block_base_.get(), s.bucket_start_.get(), s.piece_start_.get(),
+                          s.sorted_.get()};
+        off[j] = (uint32_t)blocks;
+        blocks += s.blocks_;
+        Bmax = std::max(Bmax, p.B);
+    }
+    NKR_REQUIRE(blocks < (1ull << 31), "schedule batch too large");
+    off[J] = (uint32_t)blocks;
+    NKR_HIP(hipMemcpyAsync(table_.get(), st, total, hipMemcpyHostToDevice, stream));
+    stage_.uploaded(stream);
+    const SortJob *dj = reinterpret_cast<const SortJob *>(table_.get());
+    const uint32_t *doff = reinterpret_cast<const uint32_t *>(table_.get() + jobs_bytes);
+    const size_t lds = (size_t)Bmax * sizeof(uint32_t);
+    k_zero_counts<<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(dj);
+    NKR_LAUNCH_CHECK();
+    k_hist_batch<<<(uint32_t)blocks, 256, lds, stream>>>(dj, doff, J);
+    NKR_LAUNCH_CHECK();
+    k_schedule_scan_batch<<<J, 1024, 0, stream>>>(dj);
+    NKR_LAUNCH_CHECK();
+    k_scatter_batch<<<(uint32_t)blocks, 256, lds, stream>>>(dj, doff, J);
+    NKR_LAUNCH_CHECK();
 }
 
 template <class F>
@@ -584,9 +738,15 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.c = jb.sched->plan().c;
         d.piece = jb.sched->plan().piece;
         for (uint32_t L = 0; L < kMaxFoldLevels; ++L) d.level_base[L] = level_base[j][L];
-        thread_base_[j + 1] = thread_base_[j] + (uint32_t)jb.sched->max_pieces();
+        thread_base_[j + 1] = thread_base_[j] + jb.sched->max_pieces();
     }
     thread_total_ = thread_base_[J];
+    complete_bound_ = 0;
+    for (uint32_t j = 0; j < J; ++j) complete_bound_ += ((uint64_t)host[j].B * 16 + 63) & ~uint64_t(63);
+    NKR_REQUIRE(complete_bound_ < (1ull << 32) && thread_total_ < (1ull << 32),
+                "MSM batch too large for 32-bit work indices");
+    complete_base_.ensure(J + 1);
+    complete_lanes_.ensure(J);
     jobs_dev_.ensure(sizeof(JobDev<F>) * J);
     thread_base_dev_.ensure(J + 1);  // actual piece offsets, filled on the device per run
     NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host, sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
@@ -612,7 +772,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     if (timer) timer->begin(kStageAccumulate, stream);
     k_piece_scan<F><<<1, 1024, 0, stream>>>(jd, J, thread_base_dev_.get());
     NKR_LAUNCH_CHECK();
-    const uint64_t blocks = std::min<uint64_t>(div_up(thread_total_, 256), resident_accumulate_blocks<F>());
+    const uint64_t blocks = std::min<uint64_t>(div_up(thread_total_, 256), resident_blocks<k_accumulate<F>>());
     k_accumulate<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
     NKR_LAUNCH_CHECK();
     if (timer) {
@@ -627,7 +787,13 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
         k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
     }
-    k_complete<F><<<dim3(div_up((uint64_t)Bmax * 16, 256), J), 256, 0, stream>>>(jd);
+    k_complete_scan<F><<<1, 1024, 0, stream>>>(jd, J, complete_base_.get(), complete_lanes_.get());
+    NKR_LAUNCH_CHECK();
+    {
+        const uint64_t blocks = std::min<uint64_t>(div_up(complete_bound_, 256), resident_blocks<k_complete<F>>());
+        k_complete<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(
+            jd, complete_base_.get(), complete_lanes_.get(), J);
+    }
     NKR_LAUNCH_CHECK();
     uint32_t t = 0;
     for (; t + 1 <= cmax - 1; ++t) {
