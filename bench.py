@@ -87,7 +87,7 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "pmc_traffic_prove_matmul100_sp
 
 
 def measured_traffic():
-    """HBM bytes per prove of k_accumulate (its three launches), from the
+    """HBM bytes per prove of k_accumulate (all its launches), from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload,
     scaled by the copy-kernel calibration; None if the summary is absent."""
     try:

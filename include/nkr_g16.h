@@ -258,6 +258,14 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream);    /
  * measurement. */
 int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log, uint32_t split_policy);
 
+/* MSM tuning knobs for every MSM run afterwards: the accumulation splits
+ * each bucket's entries into pieces summed by one thread, with the piece
+ * length chosen on the device so that a vector has about 2^target_pieces_log
+ * pieces (10 .. 22); at most complete_lanes_cap lanes (1, 2, 4, 8 or 16)
+ * then sum one bucket's pieces.  Set while no proof runs.  Results do not
+ * depend on either; the defaults (19, 16) are chosen by measurement. */
+int nkr_msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap);
+
 /* y = M * z for one CSR matrix (host memory). */
 int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,
              uint64_t nnz, const void *z, void *y, int device);

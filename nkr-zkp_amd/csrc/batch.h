@@ -25,6 +25,12 @@
 
 namespace nkr {
 
+// The MSM plan a batch uses for a context's plan: a fixed piece length of
+// 32 entries.  The many jobs of a batch already fill the device, so long
+// pieces (fewer partial sums to complete) pay; one proof alone lets the
+// device choose shorter ones.  Results do not depend on the piece length.
+MsmPlan batch_plan(const MsmPlan &plan);
+
 struct BatchItem {
     Groth16Prover *ctx;
     const void *witness;  // cols Fr, host memory

@@ -41,10 +41,11 @@ void parallel_for(size_t n, Fn fn)
 
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
-// Accumulation piece length for batched MSMs.
 constexpr uint32_t kBatchPiece = 32;
 
 }  // namespace
+
+MsmPlan batch_plan(const MsmPlan &plan) { return plan.with_piece(kBatchPiece); }
 
 BatchProver::BatchProver(int device) : device_(device)
 {
@@ -169,14 +170,8 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     for (size_t i = 0; i < K; ++i) {
         Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
-        // In a batch the many jobs already fill the device, so pieces are
-        // long (fewer partial sums to complete); one proof alone uses short
-        // pieces for parallelism.  Results do not depend on the piece length.
-        MsmPlan wp = g.wplan_, hp = g.hplan_;
-        wp.piece = std::max(wp.piece, kBatchPiece);
-        hp.piece = std::max(hp.piece, kBatchPiece);
-        sorts.push_back({&sl.wsched, sl.z.get(), wp});
-        sorts.push_back({&sl.hsched, sl.pa.get(), hp});
+        sorts.push_back({&sl.wsched, sl.z.get(), batch_plan(g.wplan_)});
+        sorts.push_back({&sl.hsched, sl.pa.get(), batch_plan(g.hplan_)});
     }
     sort_batch_.build(sorts, stream_);
     for (size_t i = 0; i < K; ++i) {

@@ -501,6 +501,11 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream)
     });
 }
 
+int nkr_msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap)
+{
+    return guarded([&] { msm_set_shape(target_pieces_log, complete_lanes_cap); });
+}
+
 int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log, uint32_t split_policy)
 {
     return guarded([&] {

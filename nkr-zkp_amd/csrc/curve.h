@@ -71,20 +71,27 @@ struct Xyzz {
     }
 };
 
+// The formulas below take their products two at a time (FieldOps::mul2 /
+// sqr2 / sqr_mul): independent products issued together halve the length
+// of the dependent chain a single wave waits on; the arithmetic, and so
+// every result, is that of the one-at-a-time formulas.
+
 // 2*(x, y) for an affine point (mdbl-2008-s-1).
 template <class F>
 NKR_HD Xyzz<F> xyzz_dbl_affine(const F &x, const F &y)
 {
     using O = FieldOps<F>;
     const F U = O::dbl(y);
-    const F V = O::sqr(U);
-    const F W = O::mul(U, V);
-    const F S = O::mul(x, V);
-    const F x2 = O::sqr(x);
+    F V, x2, W, S, MM;
+    O::sqr2(U, x, V, x2);
+    O::mul2(U, V, x, V, W, S);
     const F M = O::add(O::dbl(x2), x2);
+    MM = O::sqr(M);
     Xyzz<F> r;
-    r.X = O::sub(O::sqr(M), O::dbl(S));
-    r.Y = O::sub(O::mul(M, O::sub(S, r.X)), O::mul(W, y));
+    r.X = O::sub(MM, O::dbl(S));
+    F t1, t2;
+    O::mul2(M, O::sub(S, r.X), W, y, t1, t2);
+    r.Y = O::sub(t1, t2);
     r.ZZ = V;
     r.ZZZ = W;
     return r;
@@ -96,15 +103,17 @@ NKR_HD Xyzz<F> xyzz_dbl(const Xyzz<F> &p)
 {
     using O = FieldOps<F>;
     const F U = O::dbl(p.Y);
-    const F V = O::sqr(U);
-    const F W = O::mul(U, V);
-    const F S = O::mul(p.X, V);
-    const F x2 = O::sqr(p.X);
+    F V, x2, W, S, MM, ZZ3;
+    O::sqr2(U, p.X, V, x2);
+    O::mul2(U, V, p.X, V, W, S);
     const F M = O::add(O::dbl(x2), x2);
+    O::sqr_mul(M, V, p.ZZ, MM, ZZ3);
     Xyzz<F> r;
-    r.X = O::sub(O::sqr(M), O::dbl(S));
-    r.Y = O::sub(O::mul(M, O::sub(S, r.X)), O::mul(W, p.Y));
-    r.ZZ = O::mul(V, p.ZZ);
+    r.X = O::sub(MM, O::dbl(S));
+    F t1, t2;
+    O::mul2(M, O::sub(S, r.X), W, p.Y, t1, t2);
+    r.Y = O::sub(t1, t2);
+    r.ZZ = ZZ3;
     r.ZZZ = O::mul(W, p.ZZZ);
     return r;  // p = infinity gives ZZ = 0 again
 }
@@ -122,22 +131,23 @@ NKR_HD Xyzz<F> xyzz_add_affine(const Xyzz<F> &acc, const F &x2, const F &y2)
         r.ZZZ = O::one();
         return r;
     }
-    const F U2 = O::mul(x2, acc.ZZ);
-    const F S2 = O::mul(y2, acc.ZZZ);
+    F U2, S2;
+    O::mul2(x2, acc.ZZ, y2, acc.ZZZ, U2, S2);
     const F P = O::sub(U2, acc.X);
     const F R = O::sub(S2, acc.Y);
     if (P.is_zero()) {
         if (R.is_zero()) return xyzz_dbl_affine(x2, y2);
         return Xyzz<F>::infinity();
     }
-    const F PP = O::sqr(P);
-    const F PPP = O::mul(P, PP);
-    const F Qv = O::mul(acc.X, PP);
+    F PP, RR, PPP, Qv;
+    O::sqr2(P, R, PP, RR);
+    O::mul2(P, PP, acc.X, PP, PPP, Qv);
     Xyzz<F> r;
-    r.X = O::sub(O::sub(O::sqr(R), PPP), O::dbl(Qv));
-    r.Y = O::sub(O::mul(R, O::sub(Qv, r.X)), O::mul(acc.Y, PPP));
-    r.ZZ = O::mul(acc.ZZ, PP);
-    r.ZZZ = O::mul(acc.ZZZ, PPP);
+    r.X = O::sub(O::sub(RR, PPP), O::dbl(Qv));
+    F t1, t2;
+    O::mul2(R, O::sub(Qv, r.X), acc.Y, PPP, t1, t2);
+    r.Y = O::sub(t1, t2);
+    O::mul2(acc.ZZ, PP, acc.ZZZ, PPP, r.ZZ, r.ZZZ);
     return r;
 }
 
@@ -148,24 +158,25 @@ NKR_HD Xyzz<F> xyzz_add(const Xyzz<F> &p, const Xyzz<F> &q)
     using O = FieldOps<F>;
     if (p.is_inf()) return q;
     if (q.is_inf()) return p;
-    const F U1 = O::mul(p.X, q.ZZ);
-    const F U2 = O::mul(q.X, p.ZZ);
-    const F S1 = O::mul(p.Y, q.ZZZ);
-    const F S2 = O::mul(q.Y, p.ZZZ);
+    F U1, U2, S1, S2;
+    O::mul2(p.X, q.ZZ, q.X, p.ZZ, U1, U2);
+    O::mul2(p.Y, q.ZZZ, q.Y, p.ZZZ, S1, S2);
     const F P = O::sub(U2, U1);
     const F R = O::sub(S2, S1);
     if (P.is_zero()) {
         if (R.is_zero()) return xyzz_dbl(p);
         return Xyzz<F>::infinity();
     }
-    const F PP = O::sqr(P);
-    const F PPP = O::mul(P, PP);
-    const F Qv = O::mul(U1, PP);
+    F PP, ZZ12, PPP, Qv, RR, ZZZ12;
+    O::sqr_mul(P, p.ZZ, q.ZZ, PP, ZZ12);
+    O::mul2(P, PP, U1, PP, PPP, Qv);
+    O::sqr_mul(R, p.ZZZ, q.ZZZ, RR, ZZZ12);
     Xyzz<F> r;
-    r.X = O::sub(O::sub(O::sqr(R), PPP), O::dbl(Qv));
-    r.Y = O::sub(O::mul(R, O::sub(Qv, r.X)), O::mul(S1, PPP));
-    r.ZZ = O::mul(O::mul(p.ZZ, q.ZZ), PP);
-    r.ZZZ = O::mul(O::mul(p.ZZZ, q.ZZZ), PPP);
+    r.X = O::sub(O::sub(RR, PPP), O::dbl(Qv));
+    F t1, t2;
+    O::mul2(R, O::sub(Qv, r.X), S1, PPP, t1, t2);
+    r.Y = O::sub(t1, t2);
+    O::mul2(ZZ12, PP, ZZZ12, PPP, r.ZZ, r.ZZZ);
     return r;
 }
 

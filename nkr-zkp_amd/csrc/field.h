@@ -260,6 +260,71 @@ NKR_DEV Fp<P> fp_mul_fips(const Fp<P> &a, const Fp<P> &b)
     for (int j = 0; j < 8; ++j) r.l[j] = t[j];
     return fp_reduce_once(r);
 }
+
+// Two independent mac96 steps in one block: each v_mad_u64_u32 has the
+// other chain's instruction beside it while its result is pending, so one
+// wave keeps issuing where a single chain would wait on the multiplier.
+NKR_DEV void mac96x2(uint64_t &acc0, uint32_t &top0, uint32_t x0, uint32_t y0, uint64_t &acc1, uint32_t &top1,
+                     uint32_t x1, uint32_t y1)
+{
+    uint64_t c0, c1;
+    asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
+        "v_mad_u64_u32 %2, %5, %8, %9, %2\n\t"
+        "v_addc_co_u32 %1, %4, %1, 0, %4\n\t"
+        "v_addc_co_u32 %3, %5, %3, 0, %5"
+        : "+v"(acc0), "+v"(top0), "+v"(acc1), "+v"(top1), "=&s"(c0), "=&s"(c1)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
+}
+
+// Two independent Montgomery products (r0 = a0*b0, r1 = a1*b1) with their
+// column steps interleaved: the same instructions as two fp_mul_fips calls,
+// in half the dependent-chain latency.  Outputs may alias inputs.
+template <class P>
+NKR_DEV void fp_mul_fips2(const Fp<P> &a0, const Fp<P> &b0, const Fp<P> &a1, const Fp<P> &b1, Fp<P> &r0,
+                          Fp<P> &r1)
+{
+    uint32_t m0[8], m1[8], t0[8], t1[8];
+    uint64_t acc0 = 0, acc1 = 0;
+    uint32_t top0 = 0, top1 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+            mac96x2(acc0, top0, a0.l[j], b0.l[i - j], acc1, top1, a1.l[j], b1.l[i - j]);
+            mac96x2(acc0, top0, m0[j], P::P[i - j], acc1, top1, m1[j], P::P[i - j]);
+        }
+        mac96x2(acc0, top0, a0.l[i], b0.l[0], acc1, top1, a1.l[i], b1.l[0]);
+        m0[i] = (uint32_t)acc0 * P::INV;
+        m1[i] = (uint32_t)acc1 * P::INV;
+        mac96x2(acc0, top0, m0[i], P::P[0], acc1, top1, m1[i], P::P[0]);
+        acc0 = (acc0 >> 32) | ((uint64_t)top0 << 32);
+        acc1 = (acc1 >> 32) | ((uint64_t)top1 << 32);
+        top0 = top1 = 0;
+    }
+#pragma unroll
+    for (int i = 8; i < 15; ++i) {
+#pragma unroll
+        for (int j = i - 7; j < 8; ++j) {
+            mac96x2(acc0, top0, a0.l[j], b0.l[i - j], acc1, top1, a1.l[j], b1.l[i - j]);
+            mac96x2(acc0, top0, m0[j], P::P[i - j], acc1, top1, m1[j], P::P[i - j]);
+        }
+        t0[i - 8] = (uint32_t)acc0;
+        t1[i - 8] = (uint32_t)acc1;
+        acc0 = (acc0 >> 32) | ((uint64_t)top0 << 32);
+        acc1 = (acc1 >> 32) | ((uint64_t)top1 << 32);
+        top0 = top1 = 0;
+    }
+    t0[7] = (uint32_t)acc0;
+    t1[7] = (uint32_t)acc1;
+    Fp<P> x0, x1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        x0.l[j] = t0[j];
+        x1.l[j] = t1[j];
+    }
+    r0 = fp_reduce_once(x0);
+    r1 = fp_reduce_once(x1);
+}
 #endif
 
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -304,6 +369,19 @@ NKR_HD Fp<P> fp_mul(const Fp<P> &a, const Fp<P> &b)
     return fp_mul_fips(a, b);
 #else
     return fp_mul_host64(a, b);
+#endif
+}
+
+// r0 = a0*b0 and r1 = a1*b1 (independent products; outputs may alias inputs).
+template <class P>
+NKR_HD void fp_mul2(const Fp<P> &a0, const Fp<P> &b0, const Fp<P> &a1, const Fp<P> &b1, Fp<P> &r0, Fp<P> &r1)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    fp_mul_fips2(a0, b0, a1, b1, r0, r1);
+#else
+    const Fp<P> x = fp_mul_host64(a0, b0), y = fp_mul_host64(a1, b1);
+    r0 = x;
+    r1 = y;
 #endif
 }
 
@@ -381,8 +459,8 @@ NKR_HD fq2_t fq2_neg(const fq2_t &a) { return {fp_neg(a.c0), fp_neg(a.c1)}; }
 // Karatsuba: 3 base-field products.
 NKR_HD fq2_t fq2_mul(const fq2_t &a, const fq2_t &b)
 {
-    const fq_t aa = fp_mul(a.c0, b.c0);
-    const fq_t bb = fp_mul(a.c1, b.c1);
+    fq_t aa, bb;
+    fp_mul2(a.c0, b.c0, a.c1, b.c1, aa, bb);
     const fq_t s = fp_mul(fp_add(a.c0, a.c1), fp_add(b.c0, b.c1));
     return {fp_sub(aa, bb), fp_sub(fp_sub(s, aa), bb)};
 }
@@ -390,8 +468,8 @@ NKR_HD fq2_t fq2_mul(const fq2_t &a, const fq2_t &b)
 // (a0 + a1 u)^2 = (a0+a1)(a0-a1) + 2 a0 a1 u: 2 base-field products.
 NKR_HD fq2_t fq2_sqr(const fq2_t &a)
 {
-    const fq_t p = fp_mul(fp_add(a.c0, a.c1), fp_sub(a.c0, a.c1));
-    const fq_t m = fp_mul(a.c0, a.c1);
+    fq_t p, m;
+    fp_mul2(fp_add(a.c0, a.c1), fp_sub(a.c0, a.c1), a.c0, a.c1, p, m);
     return {p, fp_dbl(m)};
 }
 
@@ -414,6 +492,17 @@ struct FieldOps<fq_t> {
     NKR_HD static fq_t dbl(const fq_t &a) { return fp_dbl(a); }
     NKR_HD static fq_t neg(const fq_t &a) { return fp_neg(a); }
     NKR_HD static fq_t mul(const fq_t &a, const fq_t &b) { return fp_mul(a, b); }
+    NKR_HD static void mul2(const fq_t &a0, const fq_t &b0, const fq_t &a1, const fq_t &b1, fq_t &r0, fq_t &r1)
+    {
+        fp_mul2(a0, b0, a1, b1, r0, r1);
+    }
+    // ra = a^2, rb = b^2
+    NKR_HD static void sqr2(const fq_t &a, const fq_t &b, fq_t &ra, fq_t &rb) { fp_mul2(a, a, b, b, ra, rb); }
+    // ra = a^2, rbc = b*c
+    NKR_HD static void sqr_mul(const fq_t &a, const fq_t &b, const fq_t &c, fq_t &ra, fq_t &rbc)
+    {
+        fp_mul2(a, a, b, c, ra, rbc);
+    }
     NKR_HD static fq_t sqr(const fq_t &a) { return fp_sqr(a); }
     NKR_HD static fq_t inv(const fq_t &a) { return fp_inv(a); }
     NKR_HD static fq_t zero() { return fq_t::zero(); }
@@ -427,6 +516,27 @@ struct FieldOps<fq2_t> {
     NKR_HD static fq2_t dbl(const fq2_t &a) { return fq2_dbl(a); }
     NKR_HD static fq2_t neg(const fq2_t &a) { return fq2_neg(a); }
     NKR_HD static fq2_t mul(const fq2_t &a, const fq2_t &b) { return fq2_mul(a, b); }
+    // G2 formulas already hold many live values: the pairs are formed inside
+    // each Fq2 product (fq2_mul, fq2_sqr) instead of across two of them.
+    NKR_HD static void mul2(const fq2_t &a0, const fq2_t &b0, const fq2_t &a1, const fq2_t &b1, fq2_t &r0,
+                            fq2_t &r1)
+    {
+        const fq2_t x = fq2_mul(a0, b0), y = fq2_mul(a1, b1);
+        r0 = x;
+        r1 = y;
+    }
+    NKR_HD static void sqr2(const fq2_t &a, const fq2_t &b, fq2_t &ra, fq2_t &rb)
+    {
+        const fq2_t x = fq2_sqr(a), y = fq2_sqr(b);
+        ra = x;
+        rb = y;
+    }
+    NKR_HD static void sqr_mul(const fq2_t &a, const fq2_t &b, const fq2_t &c, fq2_t &ra, fq2_t &rbc)
+    {
+        const fq2_t x = fq2_sqr(a), y = fq2_mul(b, c);
+        ra = x;
+        rbc = y;
+    }
     NKR_HD static fq2_t sqr(const fq2_t &a) { return fq2_sqr(a); }
     NKR_HD static fq2_t inv(const fq2_t &a) { return fq2_inv(a); }
     NKR_HD static fq2_t zero() { return fq2_t::zero(); }

@@ -8,28 +8,33 @@
 //    The copies cost W*n bases of HBM (about 1 GiB for a 2^20-point G1 key),
 //    which a 288 GB device affords.
 //  * Scalars are recoded into signed c-bit digits; a counting sort groups the
-//    (window, point, sign) entries by bucket using block-local LDS histograms,
-//    one global atomic per (block, bucket) to reserve ranges, and an LDS
-//    cursor scatter -- no inter-workgroup waiting and no host round trip.
+//    (window, point, sign) entries by bucket: about one workgroup per CU
+//    histograms a contiguous slice of scalars in LDS and writes its counts as
+//    one row of a (workgroups x B) matrix, a column scan turns the matrix into
+//    per-(workgroup, bucket) offsets, and an LDS-cursor scatter places the
+//    entries -- no global atomics, no inter-workgroup waiting and no host
+//    round trip.
 //  * Several MSMs run as one batch ("jobs"): every stage below is a single
 //    launch over all jobs, so the latency of each reduction level is paid
 //    once per batch, not once per MSM.
 //      1. accumulate: one thread sums a piece of 4..64 entries of one bucket
 //         (mixed affine additions into XYZZ); the launch holds only the
 //         resident waves, which take pieces by grid stride;
-//      2. fold: buckets left with more than 256 partial sums (skewed digit
+//      2. fold: buckets left with more than 64 partial sums (skewed digit
 //         distributions, e.g. witnesses that are mostly 0 or 1) are folded
-//         16:1 until they are not; other buckets are untouched;
-//      3. complete: 16 lanes per bucket, strided partial sums then a 4-step
-//         shuffle tree, give the bucket sums S_m;
+//         4:1 until they are not; other buckets are untouched;
+//      3. complete: 1..16 lanes per bucket (about 8 partials per lane),
+//         strided partial sums then a shuffle tree, give the bucket sums S_m;
 //      4. bit planes: a log-depth tree merges bucket blocks into
-//         U_j = sum over m with bit j of S_m (c points per job).
+//         U_j = sum over m with bit j of S_m (c points per job), several
+//         levels per launch inside LDS.
 //    The host combines sum_j 2^j U_j (c doublings) and normalises.
 // The reference computes the same sums with libff multi_exp<BDLO12>
 // (third_party/.../libff/algebra/scalar_multiplication/multiexp.tcc:165-282);
 // the affine result is unique, so it matches the reference bit for bit.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "curve.h"
@@ -37,24 +42,60 @@
 
 namespace nkr {
 
-struct MsmPlan {
-    uint32_t n = 0;      // number of bases / scalars
-    uint32_t c = 0;      // window width in bits
-    uint32_t W = 0;      // windows (signed digits), c*W >= 255
-    uint32_t B = 0;      // buckets 1..B, B = 2^(c-1)
-    uint32_t piece = 0;  // entries summed serially by one accumulation thread
-    static MsmPlan choose(uint32_t n, uint32_t c_override = 0);
-    uint64_t max_entries() const { return (uint64_t)n * W; }
-};
-
 // Largest accumulation piece, fold fan-in and the completion limit.  Small
 // MSMs use shorter pieces (down to kMsmPieceMin) so that the serial chain of
 // one thread, not the amount of work, stops dominating their latency.
 constexpr uint32_t kMsmPiece = 64;
 constexpr uint32_t kMsmPieceMin = 4;
-constexpr uint32_t kMsmFoldIn = 16;
-constexpr uint32_t kMsmCompleteMax = 256;
-constexpr uint32_t kMaxFoldLevels = 8;
+constexpr uint32_t kMsmFoldIn = 4;
+constexpr uint32_t kMsmCompleteMax = 64;
+constexpr uint32_t kMaxFoldLevels = 12;
+// The piece length is chosen on the device once the number of non-zero
+// digits E is known: the largest power of two <= E / 2^target_log,
+// clamped to [piece_lo, piece_hi], so a schedule has about 2^target_log
+// pieces (default 2^19: every resident accumulation thread busy).
+constexpr uint32_t kMsmTargetPiecesLog = 19;
+// Largest bucket count (c <= 15): the bucket scans stage B counts in 64 KiB of LDS.
+constexpr uint32_t kMsmMaxBuckets = 1u << 14;
+
+// Process-wide MSM tuning knobs: the target piece count (2^target_log, 10 ..
+// 22) and the most lanes that complete one bucket (1 .. 16, a power of two).
+// Set them before proving, not while proofs run.
+void msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap);
+uint32_t msm_target_pieces_log();
+uint32_t msm_complete_lanes_cap();
+
+struct MsmPlan {
+    uint32_t n = 0;         // number of bases / scalars
+    uint32_t c = 0;         // window width in bits
+    uint32_t W = 0;         // windows (signed digits), c*W >= 255
+    uint32_t B = 0;         // buckets 1..B, B = 2^(c-1)
+    uint32_t piece_lo = 0;  // bounds of the accumulation piece length
+    uint32_t piece_hi = 0;
+    static MsmPlan choose(uint32_t n, uint32_t c_override = 0);
+    uint64_t max_entries() const { return (uint64_t)n * W; }
+    // Upper bound of level-0 pieces for any scalars when the device targets
+    // 2^target_log pieces: the piece p it picks is piece_hi, or p > E/2^(target_log+1).
+    uint64_t max_pieces(uint32_t target_log) const
+    {
+        const uint64_t by_lo = max_entries() / piece_lo;
+        const uint64_t by_target = std::max<uint64_t>(2ull << target_log, max_entries() / piece_hi);
+        return std::min(by_lo, by_target) + B + 1;
+    }
+    // The same plan with a fixed piece length.
+    MsmPlan with_piece(uint32_t piece) const
+    {
+        MsmPlan p = *this;
+        p.piece_lo = p.piece_hi = piece;
+        return p;
+    }
+};
+
+// Device bytes of one schedule (MsmSchedule) and of one MSM job's partial
+// sums and per-bucket arrays (MsmBatch) for a plan.
+uint64_t msm_schedule_bytes(const MsmPlan &plan);
+template <class F>
+uint64_t msm_job_bytes(const MsmPlan &plan);
 
 // Window-shifted affine bases of one proving-key vector.
 template <class F>
@@ -82,17 +123,31 @@ public:
     // Sizes the buffers for `plan`; returns the scalars per sort workgroup.
     uint32_t prepare(const MsmPlan &plan);
     const MsmPlan &plan() const { return plan_; }
-    uint64_t max_pieces() const { return plan_.max_entries() / plan_.piece + plan_.B + 1; }
-    // Device counters: [0] entries (non-zero digits), [1] level-0 pieces.
+    uint64_t max_pieces() const { return plan_.max_pieces(target_log_); }
+    // Device counters: [0] entries (non-zero digits), [1] level-0 pieces,
+    // [2] largest piece count of a bucket, [3] piece length.
     const uint32_t *totals() const { return bucket_start_.get() + plan_.B; }
+    // The same four counters on the host; blocks until the sort that writes
+    // them has run.
+    const uint32_t *totals_host() const
+    {
+        host_stage_->wait();
+        return host_totals_;
+    }
+    // Sort workgroups for n scalars (each histograms a contiguous slice).
+    static uint32_t sort_blocks(uint32_t n);
 
     MsmPlan plan_;
     uint32_t blocks_ = 0;
+    uint32_t target_log_ = kMsmTargetPiecesLog;  // as of the last prepare
     DeviceBuffer<uint32_t> sorted_;        // entries: sign << 31 | (w * n + i)
     DeviceBuffer<uint32_t> count_;         // per bucket (B)
-    DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the piece total at [B + 1]
+    DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the totals above
     DeviceBuffer<uint32_t> piece_start_;   // B + 1
-    DeviceBuffer<uint32_t> block_base_;    // blocks * B
+    DeviceBuffer<uint32_t> block_count_;   // blocks x B: per-workgroup counts, then offsets
+    PinnedStaging<uint32_t> own_totals_;   // host copy of the counters (build)
+    const uint32_t *host_totals_ = nullptr;
+    const PinnedStaging<uint32_t> *host_stage_ = nullptr;
 };
 
 // Many schedules with one launch per sort phase (zero, histogram, scan,
@@ -110,6 +165,8 @@ public:
 private:
     PinnedStaging<uint8_t> stage_;
     DeviceBuffer<uint8_t> table_;
+    DeviceBuffer<uint32_t> totals_dev_;
+    PinnedStaging<uint32_t> totals_host_;
 };
 
 // Stage indices used with StageTimer by the MSM code.

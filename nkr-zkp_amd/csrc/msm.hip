@@ -11,8 +11,16 @@ namespace nkr {
 namespace {
 
 constexpr uint32_t kEntryMask = 0x7fffffffu;
-constexpr uint32_t kSortBlocksMax = 1024;
-constexpr uint32_t kTreeTailThreads = 256;
+// Sort workgroups: about one per CU, each histogramming a contiguous slice of
+// at least kSortMinPerBlock scalars with up to kSortThreads threads.
+constexpr uint32_t kSortBlocksTarget = 256;
+constexpr uint32_t kSortMinPerBlock = 256;
+constexpr uint32_t kSortThreads = 1024;
+// Column scan tile: 64 buckets x kScanGroups row groups per workgroup.
+constexpr uint32_t kScanGroups = 16;
+// LDS per tree workgroup (two buffers of 2^h blocks); 64 KiB keeps two
+// workgroups per CU.
+constexpr size_t kTreeLdsBytes = 64 * 1024;
 
 template <class F>
 struct JobDev {
@@ -21,6 +29,7 @@ struct JobDev {
     const uint32_t *bucket_start;  // B + 1
     const uint32_t *piece_start;   // B + 1
     const uint32_t *piece_max;     // largest level-0 piece count of any bucket
+    const uint32_t *piece_len;     // entries per level-0 piece (chosen on the device)
     Xyzz<F> *pool;                 // partial sums (level 0 region, then fold levels)
     uint32_t *reg_start;           // B: where bucket k's partials live in `pool`
     uint32_t *reg_count;           // B
@@ -28,7 +37,7 @@ struct JobDev {
     Xyzz<F> *sums;                 // B: S_{k+1}
     Xyzz<F> *tree[2];              // B each
     Xyzz<F> *planes;               // c
-    uint32_t B, c, piece;
+    uint32_t B, c;
     uint32_t level_base[kMaxFoldLevels];  // where each fold level's partials start in `pool`
 };
 
@@ -66,17 +75,11 @@ __device__ __forceinline__ void for_each_digit(const fr_t &s, uint32_t c, uint32
     }
 }
 
-__global__ void k_zero_u32(uint32_t *p, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
-// Block-local histogram of bucket magnitudes, then one global atomic per
-// (block, bucket) reserves the block's range inside each bucket.
+// Histogram of the bucket magnitudes of scalars [b * per_block, ...) in LDS,
+// written as row b of the (blocks x B) count matrix.
 __device__ __forceinline__ void hist_block(uint32_t *hist, const fr_t *__restrict__ scalars, uint32_t n, uint32_t c,
                                           uint32_t W, uint32_t B, uint32_t per_block, uint32_t b,
-                                          uint32_t *__restrict__ count, uint32_t *__restrict__ block_base)
+                                          uint32_t *__restrict__ block_count)
 {
     for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) hist[k] = 0;
     __syncthreads();
@@ -90,20 +93,48 @@ __device__ __forceinline__ void hist_block(uint32_t *hist, const fr_t *__restric
         });
     }
     __syncthreads();
-    uint32_t *mine = block_base + (size_t)b * B;
-    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) {
-        const uint32_t h = hist[k];
-        mine[k] = h ? atomicAdd(&count[k], h) : 0u;
+    uint32_t *row = block_count + (size_t)b * B;
+    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) row[k] = hist[k];
+}
+
+// Column scan of the count matrix for buckets [k0, k0 + 64): row b of
+// block_count becomes the offset of workgroup b's entries inside each
+// bucket (the sum of the rows above it) and count[k] the column total.
+// 64 x kScanGroups threads: each row group sums its rows, the groups'
+// sums are scanned in LDS, then each group rewrites its rows as offsets.
+__device__ __forceinline__ void colscan_tile(uint32_t *__restrict__ block_count, uint32_t blocks, uint32_t B,
+                                            uint32_t k0, uint32_t *__restrict__ count)
+{
+    __shared__ uint32_t part[kScanGroups][64];
+    const uint32_t kk = threadIdx.x & 63u, g = threadIdx.x >> 6;
+    const uint32_t k = k0 + kk;
+    const uint32_t rows = (blocks + kScanGroups - 1) / kScanGroups;
+    const uint32_t r0 = min(blocks, g * rows), r1 = min(blocks, r0 + rows);
+    uint32_t sum = 0;
+    if (k < B)
+        for (uint32_t r = r0; r < r1; ++r) sum += block_count[(size_t)r * B + k];
+    part[g][kk] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t h = 0; h < g; ++h) run += part[h][kk];
+    if (k < B) {
+        for (uint32_t r = r0; r < r1; ++r) {
+            uint32_t *p = block_count + (size_t)r * B + k;
+            const uint32_t v = *p;
+            *p = run;
+            run += v;
+        }
+        if (g == kScanGroups - 1) count[k] = run;
     }
 }
 
 __device__ __forceinline__ void scatter_block(uint32_t *cursor, const fr_t *__restrict__ scalars, uint32_t n,
                                              uint32_t c, uint32_t W, uint32_t B, uint32_t per_block, uint32_t b,
                                              const uint32_t *__restrict__ bucket_start,
-                                             const uint32_t *__restrict__ block_base, uint32_t *__restrict__ sorted)
+                                             const uint32_t *__restrict__ block_off, uint32_t *__restrict__ sorted)
 {
-    const uint32_t *mine = block_base + (size_t)b * B;
-    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) cursor[k] = bucket_start[k] + mine[k];
+    const uint32_t *row = block_off + (size_t)b * B;
+    for (uint32_t k = threadIdx.x; k < B; k += blockDim.x) cursor[k] = bucket_start[k] + row[k];
     __syncthreads();
     const uint32_t beg = b * per_block;
     const uint32_t end = min(n, beg + per_block);
@@ -117,47 +148,60 @@ __device__ __forceinline__ void scatter_block(uint32_t *cursor, const fr_t *__re
     }
 }
 
-__global__ __launch_bounds__(256) void k_hist(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                                              uint32_t B, uint32_t per_block, uint32_t *__restrict__ count,
-                                              uint32_t *__restrict__ block_base)
+__global__ __launch_bounds__(kSortThreads) void k_hist(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c,
+                                                       uint32_t W, uint32_t B, uint32_t per_block,
+                                                       uint32_t *__restrict__ block_count)
 {
     extern __shared__ uint32_t lds[];
-    hist_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, count, block_base);
+    hist_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, block_count);
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c, uint32_t W,
-                                                 uint32_t B, uint32_t per_block,
-                                                 const uint32_t *__restrict__ bucket_start,
-                                                 const uint32_t *__restrict__ block_base, uint32_t *__restrict__ sorted)
+__global__ __launch_bounds__(64 * kScanGroups) void k_colscan(uint32_t *__restrict__ block_count, uint32_t blocks,
+                                                              uint32_t B, uint32_t *__restrict__ count)
+{
+    colscan_tile(block_count, blocks, B, blockIdx.x * 64, count);
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scatter(const fr_t *__restrict__ scalars, uint32_t n, uint32_t c,
+                                                          uint32_t W, uint32_t B, uint32_t per_block,
+                                                          const uint32_t *__restrict__ bucket_start,
+                                                          const uint32_t *__restrict__ block_off,
+                                                          uint32_t *__restrict__ sorted)
 {
     extern __shared__ uint32_t lds[];
-    scatter_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, bucket_start, block_base, sorted);
+    scatter_block(lds, scalars, n, c, W, B, per_block, blockIdx.x, bucket_start, block_off, sorted);
 }
 
 
-// Exclusive prefix of `len` values into out[0..len] within one workgroup.
+// Exclusive prefix of `len` (<= kMsmMaxBuckets) values into out[0..len]
+// within one workgroup; returns the total.  The values are staged in LDS
+// with coalesced loads, each thread scans a run of `per` of them, and the
+// runs' sums are scanned across the workgroup.  Runs sit at an odd LDS
+// stride, so the threads of a wave walking their runs hit distinct banks.
 //   mode 0: v[k] = src[k]
 //   mode 2: v[k] = ceil(src[k] / T)
 //   mode 3: v[k] = src[k] > lim ? ceil(src[k] / T) : 0
-__device__ void block_scan(const uint32_t *__restrict__ src, uint32_t len, uint32_t mode, uint32_t T, uint32_t lim,
-                           uint32_t *__restrict__ out, uint32_t *__restrict__ max_out = nullptr)
+__device__ uint32_t block_scan(const uint32_t *__restrict__ src, uint32_t len, uint32_t mode, uint32_t T,
+                               uint32_t lim, uint32_t *__restrict__ out, uint32_t *__restrict__ max_out = nullptr)
 {
+    __shared__ uint32_t vals[kMsmMaxBuckets + 1024];
     __shared__ uint32_t part[1024];
     __shared__ uint32_t pmax;
-    if (threadIdx.x == 0) pmax = 0;
-    __syncthreads();
-    auto value = [&](uint32_t k) -> uint32_t {
-        const uint32_t x = src[k];
-        if (mode == 0) return x;
-        if (mode == 2) return (x + T - 1) / T;
-        return x > lim ? (x + T - 1) / T : 0u;
-    };
     const uint32_t per = (len + blockDim.x - 1) / blockDim.x;
+    const uint32_t stride = per | 1u;
+    auto at = [&](uint32_t k) -> uint32_t { return (k / per) * stride + k % per; };
+    if (threadIdx.x == 0) pmax = 0;
+    for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) {
+        const uint32_t x = src[k];
+        vals[at(k)] = mode == 0 ? x : (mode == 2 || x > lim) ? (x + T - 1) / T : 0u;
+    }
+    __syncthreads();
     const uint32_t beg = min(len, threadIdx.x * per);
     const uint32_t end = min(len, beg + per);
+    uint32_t *run_vals = vals + threadIdx.x * stride;
     uint32_t sum = 0, mx = 0;
-    for (uint32_t k = beg; k < end; ++k) {
-        const uint32_t v = value(k);
+    for (uint32_t k = 0; k < end - beg; ++k) {
+        const uint32_t v = run_vals[k];
         sum += v;
         mx = max(mx, v);
     }
@@ -171,23 +215,47 @@ __device__ void block_scan(const uint32_t *__restrict__ src, uint32_t len, uint3
         __syncthreads();
     }
     uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t k = beg; k < end; ++k) {
-        out[k] = run;
-        run += value(k);
+    for (uint32_t k = 0; k < end - beg; ++k) {
+        const uint32_t v = run_vals[k];
+        run_vals[k] = run;
+        run += v;
     }
-    if (threadIdx.x == blockDim.x - 1) out[len] = part[threadIdx.x];
-    if (max_out && threadIdx.x == 0) *max_out = pmax;
     __syncthreads();
+    for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) out[k] = vals[at(k)];
+    const uint32_t total = part[blockDim.x - 1];
+    if (threadIdx.x == 0) {
+        out[len] = total;
+        if (max_out) *max_out = pmax;
+    }
+    __syncthreads();
+    return total;
 }
 
-// Bucket starts (mode 0 over counts) and level-0 piece starts (ceil(count/piece)).
+// Bucket starts (prefix of counts), the piece length for the number of
+// entries, and level-0 piece starts (prefix of ceil(count / piece)):
+// bucket_start[B + 1] = pieces, [B + 2] = largest piece count of a bucket,
+// [B + 3] = piece length.
+__device__ void schedule_scan(const uint32_t *__restrict__ count, uint32_t B, uint32_t piece_lo, uint32_t piece_hi,
+                              uint32_t target_log, uint32_t *__restrict__ bucket_start,
+                              uint32_t *__restrict__ piece_start)
+{
+    const uint32_t entries = block_scan(count, B, 0, 1, 0, bucket_start);
+    const uint32_t q = entries >> target_log;
+    uint32_t piece = piece_lo;
+    while (piece < piece_hi && 2 * piece <= q) piece <<= 1;
+    const uint32_t pieces = block_scan(count, B, 2, piece, 0, piece_start, bucket_start + B + 2);
+    if (threadIdx.x == 0) {
+        bucket_start[B + 1] = pieces;
+        bucket_start[B + 3] = piece;
+    }
+}
+
 __global__ __launch_bounds__(1024) void k_schedule_scan(const uint32_t *__restrict__ count, uint32_t B,
-                                                        uint32_t piece, uint32_t *__restrict__ bucket_start,
+                                                        uint32_t piece_lo, uint32_t piece_hi, uint32_t target_log,
+                                                        uint32_t *__restrict__ bucket_start,
                                                         uint32_t *__restrict__ piece_start)
 {
-    block_scan(count, B, 0, 1, 0, bucket_start);
-    block_scan(count, B, 2, piece, 0, piece_start, bucket_start + B + 2);
-    if (threadIdx.x == 0) bucket_start[B + 1] = piece_start[B];
+    schedule_scan(count, B, piece_lo, piece_hi, target_log, bucket_start, piece_start);
 }
 
 // Last index k in [0, len) with starts[k] <= p (starts[0] <= p < starts[len]).
@@ -207,44 +275,51 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t *starts, uint32_
 // One schedule of a batched sort.
 struct SortJob {
     const fr_t *scalars;
-    uint32_t n, c, W, B, per_block, piece;
-    uint32_t *count, *block_base, *bucket_start, *piece_start, *sorted;
+    uint32_t n, c, W, B, blocks, per_block, piece_lo, piece_hi, target_log;
+    uint32_t *count, *block_count, *bucket_start, *piece_start, *sorted;
 };
 
-__global__ __launch_bounds__(256) void k_zero_counts(const SortJob *__restrict__ jobs)
-{
-    const SortJob &jb = jobs[blockIdx.y];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < jb.B) jb.count[k] = 0;
-}
-
 // Workgroup blockIdx.x is sort block (blockIdx.x - block_off[j]) of job j.
-__global__ __launch_bounds__(256) void k_hist_batch(const SortJob *__restrict__ jobs,
-                                                    const uint32_t *__restrict__ block_off, uint32_t J)
+__global__ __launch_bounds__(kSortThreads) void k_hist_batch(const SortJob *__restrict__ jobs,
+                                                             const uint32_t *__restrict__ block_off, uint32_t J)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t j = find_segment(block_off, J, blockIdx.x);
     const SortJob &jb = jobs[j];
-    hist_block(lds, jb.scalars, jb.n, jb.c, jb.W, jb.B, jb.per_block, blockIdx.x - block_off[j], jb.count,
-               jb.block_base);
+    hist_block(lds, jb.scalars, jb.n, jb.c, jb.W, jb.B, jb.per_block, blockIdx.x - block_off[j], jb.block_count);
+}
+
+__global__ __launch_bounds__(64 * kScanGroups) void k_colscan_batch(const SortJob *__restrict__ jobs)
+{
+    const SortJob &jb = jobs[blockIdx.y];
+    if (blockIdx.x * 64 >= jb.B) return;
+    colscan_tile(jb.block_count, jb.blocks, jb.B, blockIdx.x * 64, jb.count);
 }
 
 __global__ __launch_bounds__(1024) void k_schedule_scan_batch(const SortJob *__restrict__ jobs)
 {
     const SortJob &jb = jobs[blockIdx.x];
-    block_scan(jb.count, jb.B, 0, 1, 0, jb.bucket_start);
-    block_scan(jb.count, jb.B, 2, jb.piece, 0, jb.piece_start, jb.bucket_start + jb.B + 2);
-    if (threadIdx.x == 0) jb.bucket_start[jb.B + 1] = jb.piece_start[jb.B];
+    schedule_scan(jb.count, jb.B, jb.piece_lo, jb.piece_hi, jb.target_log, jb.bucket_start, jb.piece_start);
 }
 
-__global__ __launch_bounds__(256) void k_scatter_batch(const SortJob *__restrict__ jobs,
-                                                       const uint32_t *__restrict__ block_off, uint32_t J)
+// out[4j + i] = schedule j's counter i (entries, pieces, largest piece count, piece length).
+__global__ __launch_bounds__(256) void k_gather_totals(const SortJob *__restrict__ jobs, uint32_t J,
+                                                       uint32_t *__restrict__ out)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 4 * J) return;
+    const SortJob &jb = jobs[g >> 2];
+    out[g] = jb.bucket_start[jb.B + (g & 3u)];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_scatter_batch(const SortJob *__restrict__ jobs,
+                                                                const uint32_t *__restrict__ block_off, uint32_t J)
 {
     extern __shared__ uint32_t lds[];
     const uint32_t j = find_segment(block_off, J, blockIdx.x);
     const SortJob &jb = jobs[j];
     scatter_block(lds, jb.scalars, jb.n, jb.c, jb.W, jb.B, jb.per_block, blockIdx.x - block_off[j],
-                  jb.bucket_start, jb.block_base, jb.sorted);
+                  jb.bucket_start, jb.block_count, jb.sorted);
 }
 
 template <class F>
@@ -284,9 +359,9 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const JobDev<F> *__restrict
     if (threadIdx.x == blockDim.x - 1) piece_base[J] = part[threadIdx.x];
 }
 
-// Level 0 over all jobs: a thread sums `piece` signed bases of one bucket
-// into one partial, then moves on by the grid stride, so the launch can be
-// sized to the resident waves and every wave gets the same share of pieces.
+// Level 0 over all jobs: a thread sums one piece of signed bases of one
+// bucket into one partial, then moves on by the grid stride, so the launch
+// can be sized to the resident waves and every wave gets the same share.
 template <class F>
 __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
                                                     const uint32_t *__restrict__ piece_base, uint32_t J)
@@ -299,8 +374,9 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
         const uint32_t p = g - piece_base[j];
         const uint32_t k = find_segment(jb.piece_start, jb.B, p);
         const uint32_t q = p - jb.piece_start[k];
-        const uint32_t beg = jb.bucket_start[k] + q * jb.piece;
-        const uint32_t end = min(beg + jb.piece, jb.bucket_start[k + 1]);
+        const uint32_t piece = *jb.piece_len;
+        const uint32_t beg = jb.bucket_start[k] + q * piece;
+        const uint32_t end = min(beg + piece, jb.bucket_start[k + 1]);
         const Affine<F> *__restrict__ bases = jb.bases;
         const uint32_t *__restrict__ sorted = jb.sorted;
         Xyzz<F> acc = Xyzz<F>::infinity();
@@ -382,19 +458,21 @@ __device__ __forceinline__ Xyzz<F> shfl_xor_point(const Xyzz<F> &p, int mask)
 
 // Lanes per bucket for each job's completion and the job offsets of the
 // completion index space.  A bucket with m partials summed by L lanes costs
-// ceil(m / L) - 1 serial additions plus log2 L shuffle levels; L =
-// nextpow2(m) / 2 (1 .. 16) minimises that for the job's largest bucket
-// (m <= kMsmCompleteMax after folding).  Jobs are padded to whole waves so
-// that every wave completes buckets of one job with one L.
+// ceil(m / L) - 1 serial additions plus log2 L shuffle levels, and every lane
+// of a wave pays each shuffle level; L = nextpow2(m / 8) (1 .. 16) for the
+// job's largest bucket (m <= kMsmCompleteMax after folding) keeps the serial
+// part short without lanes that mostly add infinity.  Jobs are padded to
+// whole waves so that every wave completes buckets of one job with one L.
 template <class F>
 __global__ __launch_bounds__(1024) void k_complete_scan(const JobDev<F> *__restrict__ jobs, uint32_t J,
-                                                        uint32_t *__restrict__ base, uint32_t *__restrict__ lanes)
+                                                        uint32_t lanes_cap, uint32_t *__restrict__ base,
+                                                        uint32_t *__restrict__ lanes)
 {
     __shared__ uint32_t part[1024];
     auto job_lanes = [&](uint32_t j) -> uint32_t {
         const uint32_t m = min(*jobs[j].piece_max, kMsmCompleteMax);
         uint32_t L = 1;
-        while (2 * L < m && L < 16) L <<= 1;  // nextpow2(m) / 2
+        while (8 * L < m && L < lanes_cap) L <<= 1;  // about 8 partials per lane
         return L;
     };
     auto job_span = [&](uint32_t j, uint32_t L) -> uint32_t { return (jobs[j].B * L + 63u) & ~63u; };
@@ -446,63 +524,70 @@ __global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ 
     }
 }
 
-// Bit-plane tree, level t -> t+1.  A level-t block covers buckets
-// [b*2^t, (b+1)*2^t) and holds P_0..P_{t-1} (sums of its buckets with that
-// bit set) and T (its total).  Level 0 reads S_m (S_0 = infinity) from sums.
-// The last level writes U_j = P_j, and U_{c-1} = S_B, to the planes.
+// Bit-plane tree.  A level-t block covers buckets [b*2^t, (b+1)*2^t) and
+// holds P_0..P_{t-1} (the sums of its buckets with that bit set) and T (its
+// total); level 0 is S_m itself (S_0 = infinity).  Merging blocks 2b and
+// 2b+1 into level t+1: P_e = P_e(2b) + P_e(2b+1) for e < t, P_t = T(2b+1),
+// T = T(2b) + T(2b+1).  At the last level (t+1 = c-1) the merged P_e are the
+// bit planes U_e, and U_{c-1} = S_B (bucket B = 2^(c-1) alone has bit c-1).
+//
+// One launch runs up to h levels from level t0: a workgroup loads 2^h
+// level-t0 blocks of one job into LDS and merges them level by level with a
+// barrier in between (one addition of latency per level, no launch or
+// global round trip), then stores the merged block to the other global
+// tree buffer (`src` names the one it reads).  Jobs whose tree ends earlier
+// run fewer levels; jobs already done return.
 template <class F>
-__global__ __launch_bounds__(256) void k_tree(const JobDev<F> *__restrict__ jobs, uint32_t t)
+__global__ __launch_bounds__(256) void k_tree_lds(const JobDev<F> *__restrict__ jobs, uint32_t t0, uint32_t h,
+                                                  uint32_t src)
 {
+    extern __shared__ uint8_t tree_lds[];
     const JobDev<F> &jb = jobs[blockIdx.y];
-    if (t + 1 > jb.c - 1) return;
-    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (jb.B >> (t + 1)) * (t + 2)) return;
-    tree_merge(jb, t, idx);
-}
+    if (t0 + 1 > jb.c - 1) return;
+    const uint32_t lv = min(h, jb.c - 1 - t0);
+    if (blockIdx.x >= (jb.B >> (t0 + lv))) return;
+    const uint32_t cap = (1u << h) * (t0 + 1);
+    Xyzz<F> *buf[2] = {reinterpret_cast<Xyzz<F> *>(tree_lds), reinterpret_cast<Xyzz<F> *>(tree_lds) + cap};
 
-// One merge of the bit-plane tree (level t -> t+1) for output index idx.
-template <class F>
-__device__ __forceinline__ void tree_merge(const JobDev<F> &jb, uint32_t t, uint32_t idx)
-{
-    const uint32_t E = t + 2;
-    const uint32_t b = idx / E, e = idx % E;
-    const bool last = (t + 1 == jb.c - 1);
-    auto in = [&](uint32_t blk, uint32_t ent) -> Xyzz<F> {
-        if (t == 0) return blk == 0 ? Xyzz<F>::infinity() : jb.sums[blk - 1];
-        return jb.tree[t & 1][(size_t)blk * (t + 1) + ent];
-    };
-    Xyzz<F> out;
-    if (e < t) {
-        out = xyzz_add(in(2 * b, e), in(2 * b + 1, e));
-    } else if (e == t) {
-        out = in(2 * b + 1, t);
-    } else {
-        if (last) {
-            jb.planes[jb.c - 1] = jb.sums[jb.B - 1];
-            return;
-        }
-        out = xyzz_add(in(2 * b, t), in(2 * b + 1, t));
+    const uint32_t nin = (1u << lv) * (t0 + 1);
+    const uint32_t blk0 = blockIdx.x << lv;
+    for (uint32_t idx = threadIdx.x; idx < nin; idx += blockDim.x) {
+        const uint32_t blk = blk0 + idx / (t0 + 1), ent = idx % (t0 + 1);
+        if (t0 == 0)
+            buf[0][idx] = blk == 0 ? Xyzz<F>::infinity() : jb.sums[blk - 1];
+        else
+            buf[0][idx] = jb.tree[src][(size_t)blk * (t0 + 1) + ent];
     }
-    if (last)
-        jb.planes[e] = out;
-    else
-        jb.tree[(t + 1) & 1][(size_t)b * E + e] = out;
-}
-
-// The remaining tree levels t0 .. c-2 of one job in one workgroup (blockIdx.x
-// = job), with a barrier between levels instead of a launch.  256 threads: a
-// G2 addition needs more than the 128 registers a 1024-thread workgroup
-// leaves per lane, and spilling it costs more than the launches it saves.
-template <class F>
-__global__ __launch_bounds__(kTreeTailThreads) void k_tree_tail(const JobDev<F> *__restrict__ jobs, uint32_t t0)
-{
-    const JobDev<F> &jb = jobs[blockIdx.x];
-    for (uint32_t t = t0; t + 1 <= jb.c - 1; ++t) {
-        const uint32_t work = (jb.B >> (t + 1)) * (t + 2);
-        for (uint32_t idx = threadIdx.x; idx < work; idx += blockDim.x) tree_merge(jb, t, idx);
-        __threadfence_block();
+    __syncthreads();
+    for (uint32_t s = 0; s < lv; ++s) {
+        const uint32_t t = t0 + s, E = t + 2;
+        const bool last = (t + 1 == jb.c - 1);
+        const Xyzz<F> *in = buf[s & 1];
+        Xyzz<F> *out = buf[(s + 1) & 1];
+        const uint32_t nout = (1u << (lv - s - 1)) * E;
+        for (uint32_t idx = threadIdx.x; idx < nout; idx += blockDim.x) {
+            const uint32_t b = idx / E, e = idx % E;
+            const Xyzz<F> *l = in + (size_t)(2 * b) * (t + 1);
+            const Xyzz<F> *r = l + (t + 1);
+            // one addition site for every lane: the P_e and T merges do not
+            // diverge into two serial additions
+            const uint32_t q = e < t ? e : t;
+            Xyzz<F> o = xyzz_add(l[q], r[q]);
+            if (e == t)
+                o = r[t];
+            else if (e > t && last)
+                o = jb.sums[jb.B - 1];
+            if (last)
+                jb.planes[e] = o;
+            else
+                out[idx] = o;
+        }
         __syncthreads();
     }
+    const uint32_t t1 = t0 + lv;
+    if (t1 < jb.c - 1)
+        for (uint32_t e = threadIdx.x; e <= t1; e += blockDim.x)
+            jb.tree[src ^ 1u][(size_t)blockIdx.x * (t1 + 1) + e] = buf[lv & 1][e];
 }
 
 // Base preparation: normalise the reference-layout Jacobian point and store
@@ -545,7 +630,23 @@ uint64_t resident_blocks()
     return blocks;
 }
 
+uint32_t g_target_log = kMsmTargetPiecesLog;
+uint32_t g_lanes_cap = 16;
+
 }  // namespace
+
+void msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap)
+{
+    NKR_REQUIRE(target_pieces_log >= 10 && target_pieces_log <= 22, "MSM target pieces must be 2^10 .. 2^22");
+    NKR_REQUIRE(complete_lanes_cap >= 1 && complete_lanes_cap <= 16 &&
+                    (complete_lanes_cap & (complete_lanes_cap - 1)) == 0,
+                "MSM completion lanes must be a power of two in 1 .. 16");
+    g_target_log = target_pieces_log;
+    g_lanes_cap = complete_lanes_cap;
+}
+
+uint32_t msm_target_pieces_log() { return g_target_log; }
+uint32_t msm_complete_lanes_cap() { return g_lanes_cap; }
 
 MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
 {
@@ -572,13 +673,35 @@ MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.B = 1u << (c - 1);
-    // pieces of up to n*W / 2^19 entries: several per resident thread, so the
-    // grid-stride accumulation stays balanced, and short chains for small MSMs
-    const uint64_t want = p.max_entries() >> 19;
-    p.piece = kMsmPieceMin;
-    while (p.piece < kMsmPiece && 2 * p.piece <= want) p.piece <<= 1;
+    // The piece length follows the actual number of non-zero digits (chosen
+    // on the device): sparse witnesses get short serial chains.
+    p.piece_lo = kMsmPieceMin;
+    p.piece_hi = kMsmPiece;
     NKR_REQUIRE((uint64_t)p.W * n < (1ull << 31), "MSM too large for 31-bit entry indices");
     return p;
+}
+
+uint32_t MsmSchedule::sort_blocks(uint32_t n)
+{
+    return std::max<uint32_t>(1, std::min<uint32_t>(kSortBlocksTarget, div_up(n, kSortMinPerBlock)));
+}
+
+uint64_t msm_schedule_bytes(const MsmPlan &p)
+{
+    const uint64_t blocks = MsmSchedule::sort_blocks(p.n);
+    return p.max_entries() * 4 + (uint64_t)p.B * 4 * 3 + blocks * p.B * 4 + 32;
+}
+
+template <class F>
+uint64_t msm_job_bytes(const MsmPlan &p)
+{
+    uint64_t cap = p.max_pieces(msm_target_pieces_log()), cnt = cap, total = cap;
+    while (cnt > kMsmCompleteMax) {
+        cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
+        cap = cap / kMsmFoldIn + p.B + 1;
+        total += cap;
+    }
+    return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12) + p.c * sizeof(Xyzz<F>);
 }
 
 template <class F>
@@ -594,14 +717,17 @@ void MsmBases<F>::prepare(const Jacobian<F> *jacobian, const uint32_t *perm, con
 
 uint32_t MsmSchedule::prepare(const MsmPlan &plan)
 {
+    NKR_REQUIRE(plan.B <= kMsmMaxBuckets, "MSM window too wide for the bucket scans");
+    NKR_REQUIRE(plan.piece_lo >= 1 && plan.piece_lo <= plan.piece_hi, "MSM piece bounds");
     plan_ = plan;
+    target_log_ = msm_target_pieces_log();
     const uint32_t n = plan.n, B = plan.B;
-    blocks_ = std::min<uint32_t>(kSortBlocksMax, div_up(n, 256));
+    blocks_ = sort_blocks(n);
     sorted_.ensure((size_t)n * plan.W);
     count_.ensure(B);
-    bucket_start_.ensure(B + 3);  // + total pieces, + largest piece count of a bucket
+    bucket_start_.ensure(B + 4);  // + pieces, largest piece count of a bucket, piece length
     piece_start_.ensure(B + 1);
-    block_base_.ensure((size_t)blocks_ * B);
+    block_count_.ensure((size_t)blocks_ * B);
     return div_up(n, blocks_);
 }
 
@@ -610,16 +736,22 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
     const uint32_t per_block = prepare(plan);
     const uint32_t n = plan.n, B = plan.B;
     const size_t lds = (size_t)B * sizeof(uint32_t);
+    const uint32_t threads = std::min<uint32_t>(kSortThreads, (per_block + 63) & ~63u);
     if (timer) timer->begin(kStageSort, stream);
-    k_zero_u32<<<div_up(B, 256), 256, 0, stream>>>(count_.get(), B);
+    k_hist<<<blocks_, threads, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, block_count_.get());
     NKR_LAUNCH_CHECK();
-    k_hist<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, count_.get(),
-                                          block_base_.get());
+    k_colscan<<<div_up(B, 64), 64 * kScanGroups, 0, stream>>>(block_count_.get(), blocks_, B, count_.get());
     NKR_LAUNCH_CHECK();
-    k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, plan.piece, bucket_start_.get(), piece_start_.get());
+    k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, plan.piece_lo, plan.piece_hi, target_log_,
+                                            bucket_start_.get(), piece_start_.get());
     NKR_LAUNCH_CHECK();
-    k_scatter<<<blocks_, 256, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block, bucket_start_.get(),
-                                             block_base_.get(), sorted_.get());
+    uint32_t *th = own_totals_.reuse(4);
+    NKR_HIP(hipMemcpyAsync(th, bucket_start_.get() + B, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    own_totals_.uploaded(stream);
+    host_totals_ = th;
+    host_stage_ = &own_totals_;
+    k_scatter<<<blocks_, threads, lds, stream>>>(scalars_mont, n, plan.c, plan.W, B, per_block,
+                                                 bucket_start_.get(), block_count_.get(), sorted_.get());
     NKR_LAUNCH_CHECK();
     if (timer) timer->end(kStageSort, stream);
 }
@@ -635,18 +767,19 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     table_.ensure(total);
     SortJob *jobs = reinterpret_cast<SortJob *>(st);
     uint32_t *off = reinterpret_cast<uint32_t *>(st + jobs_bytes);
-    uint32_t Bmax = 0;
+    uint32_t Bmax = 0, threads = 64;
     uint64_t blocks = 0;
     for (uint32_t j = 0; j < J; ++j) {
         MsmSchedule &s = *items[j].sched;
         const MsmPlan &p = items[j].plan;
         const uint32_t per_block = s.prepare(p);
-        jobs[j] = SortJob{items[j].scalars_mont, p.n, p.c, p.W, p.B, per_block, p.piece,
-                          s.count_.get(), s.block_base_.get(), s.bucket_start_.get(), s.piece_start_.get(),
+        jobs[j] = SortJob{items[j].scalars_mont, p.n, p.c, p.W, p.B, s.blocks_, per_block, p.piece_lo, p.piece_hi,
+                          s.target_log_, s.count_.get(), s.block_count_.get(), s.bucket_start_.get(), s.piece_start_.get(),
                           s.sorted_.get()};
         off[j] = (uint32_t)blocks;
         blocks += s.blocks_;
         Bmax = std::max(Bmax, p.B);
+        threads = std::max(threads, std::min<uint32_t>(kSortThreads, (per_block + 63) & ~63u));
     }
     NKR_REQUIRE(blocks < (1ull << 31), "schedule batch too large");
     off[J] = (uint32_t)blocks;
@@ -655,13 +788,24 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     const SortJob *dj = reinterpret_cast<const SortJob *>(table_.get());
     const uint32_t *doff = reinterpret_cast<const uint32_t *>(table_.get() + jobs_bytes);
     const size_t lds = (size_t)Bmax * sizeof(uint32_t);
-    k_zero_counts<<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(dj);
+    k_hist_batch<<<(uint32_t)blocks, threads, lds, stream>>>(dj, doff, J);
     NKR_LAUNCH_CHECK();
-    k_hist_batch<<<(uint32_t)blocks, 256, lds, stream>>>(dj, doff, J);
+    k_colscan_batch<<<dim3(div_up(Bmax, 64), J), 64 * kScanGroups, 0, stream>>>(dj);
     NKR_LAUNCH_CHECK();
     k_schedule_scan_batch<<<J, 1024, 0, stream>>>(dj);
     NKR_LAUNCH_CHECK();
-    k_scatter_batch<<<(uint32_t)blocks, 256, lds, stream>>>(dj, doff, J);
+    // every schedule's four counters to the host in one copy
+    totals_dev_.ensure(4 * J);
+    k_gather_totals<<<div_up(4 * J, 256), 256, 0, stream>>>(dj, J, totals_dev_.get());
+    NKR_LAUNCH_CHECK();
+    uint32_t *th = totals_host_.reuse(4 * J);
+    NKR_HIP(hipMemcpyAsync(th, totals_dev_.get(), 4 * J * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    totals_host_.uploaded(stream);
+    for (uint32_t j = 0; j < J; ++j) {
+        items[j].sched->host_totals_ = th + 4 * j;
+        items[j].sched->host_stage_ = &totals_host_;
+    }
+    k_scatter_batch<<<(uint32_t)blocks, threads, lds, stream>>>(dj, doff, J);
     NKR_LAUNCH_CHECK();
 }
 
@@ -675,6 +819,7 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         key.push_back(jb.sched->bucket_start_.get());
         key.push_back(jb.sched->piece_start_.get());
         key.push_back(jb.planes);
+        key.push_back(reinterpret_cast<const void *>((uintptr_t)jb.sched->max_pieces()));
     }
     if (key == key_) return;
     const uint32_t J = (uint32_t)jobs.size();
@@ -684,11 +829,14 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
     std::vector<std::array<uint32_t, kMaxFoldLevels>> level_base(J);
     fold_levels_ = 0;
     fold_cap_.assign(kMaxFoldLevels, 0);
+    thread_base_.assign(J + 1, 0);
     for (uint32_t j = 0; j < J; ++j) {
         const MsmPlan &p = jobs[j].sched->plan();
         NKR_REQUIRE(jobs[j].bases->plan().n == p.n && jobs[j].bases->plan().c == p.c,
                     "MSM bases and schedule disagree");
-        uint64_t cap = jobs[j].sched->max_pieces(), cnt = cap, base = cap;
+        const uint64_t pieces = jobs[j].sched->max_pieces();
+        thread_base_[j + 1] = thread_base_[j] + pieces;
+        uint64_t cap = pieces, cnt = pieces, base = pieces;
         uint32_t levels = 0;
         while (cnt > kMsmCompleteMax) {
             NKR_REQUIRE(levels < kMaxFoldLevels, "MSM needs too many fold levels");
@@ -717,7 +865,6 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
 
     static_assert(std::is_trivially_copyable<JobDev<F>>::value, "job records are raw-copied");
     JobDev<F> *host = reinterpret_cast<JobDev<F> *>(jobs_stage_.reuse(sizeof(JobDev<F>) * J));
-    thread_base_.assign(J + 1, 0);
     for (uint32_t j = 0; j < J; ++j) {
         const MsmJob<F> &jb = jobs[j];
         JobDev<F> &d = host[j];
@@ -726,6 +873,7 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.bucket_start = jb.sched->bucket_start_.get();
         d.piece_start = jb.sched->piece_start_.get();
         d.piece_max = jb.sched->bucket_start_.get() + jb.sched->plan().B + 2;
+        d.piece_len = jb.sched->bucket_start_.get() + jb.sched->plan().B + 3;
         d.pool = pool_.get() + pool_off[j];
         d.reg_start = reg_start_.get() + b_off[j];
         d.reg_count = reg_count_.get() + b_off[j];
@@ -736,9 +884,7 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.planes = jb.planes;
         d.B = jb.sched->plan().B;
         d.c = jb.sched->plan().c;
-        d.piece = jb.sched->plan().piece;
         for (uint32_t L = 0; L < kMaxFoldLevels; ++L) d.level_base[L] = level_base[j][L];
-        thread_base_[j + 1] = thread_base_[j] + jb.sched->max_pieces();
     }
     thread_total_ = thread_base_[J];
     complete_bound_ = 0;
@@ -779,7 +925,21 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
         timer->end(kStageAccumulate, stream);
         timer->begin(kStageFold, stream);
     }
-    for (uint32_t L = 0; L < fold_levels_; ++L) {
+    // Fold levels the largest bucket of any job needs: read from the host
+    // copy of the schedules' counters.  The accumulation is already queued,
+    // so this wait overlaps the GPU's work instead of stalling it.
+    uint32_t folds = 0;
+    for (const auto &jb : jobs) {
+        uint64_t cnt = jb.sched->totals_host()[2];
+        uint32_t levels = 0;
+        while (cnt > kMsmCompleteMax) {
+            cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
+            ++levels;
+        }
+        folds = std::max(folds, levels);
+    }
+    NKR_REQUIRE(folds <= fold_levels_, "MSM bucket exceeds the partial-sum capacity");
+    for (uint32_t L = 0; L < folds; ++L) {
         k_fold_scan<F><<<J, 1024, 0, stream>>>(jd);
         NKR_LAUNCH_CHECK();
         k_fold<F><<<dim3(div_up(fold_cap_[L], 256), J), 256, 0, stream>>>(jd, L);
@@ -787,7 +947,8 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
         k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
     }
-    k_complete_scan<F><<<1, 1024, 0, stream>>>(jd, J, complete_base_.get(), complete_lanes_.get());
+    k_complete_scan<F><<<1, 1024, 0, stream>>>(jd, J, msm_complete_lanes_cap(), complete_base_.get(),
+                                               complete_lanes_.get());
     NKR_LAUNCH_CHECK();
     {
         const uint64_t blocks = std::min<uint64_t>(div_up(complete_bound_, 256), resident_blocks<k_complete<F>>());
@@ -795,16 +956,15 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
             jd, complete_base_.get(), complete_lanes_.get(), J);
     }
     NKR_LAUNCH_CHECK();
-    uint32_t t = 0;
-    for (; t + 1 <= cmax - 1; ++t) {
-        const uint64_t work = (uint64_t)(Bmax >> (t + 1)) * (t + 2);
-        if (work <= kTreeTailThreads) break;  // the rest fits one workgroup per job
-        k_tree<F><<<dim3(div_up(work, 256), J), 256, 0, stream>>>(jd, t);
+    // bit-plane tree: as many levels per launch as two LDS buffers of 2^h
+    // level-t blocks allow
+    for (uint32_t t = 0, src = 0; t + 1 <= cmax - 1; src ^= 1u) {
+        uint32_t h = 1;
+        while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(Xyzz<F>) <= kTreeLdsBytes) ++h;
+        const size_t lds = 2 * (size_t(1) << h) * (t + 1) * sizeof(Xyzz<F>);
+        k_tree_lds<F><<<dim3(Bmax >> (t + h), J), 256, lds, stream>>>(jd, t, h, src);
         NKR_LAUNCH_CHECK();
-    }
-    if (t + 1 <= cmax - 1) {
-        k_tree_tail<F><<<J, kTreeTailThreads, 0, stream>>>(jd, t);
-        NKR_LAUNCH_CHECK();
+        t += h;
     }
     if (timer) timer->end(kStageFold, stream);
 }
@@ -848,5 +1008,7 @@ template class MsmContext<fq_t>;
 template class MsmContext<fq2_t>;
 template Jacobian<fq_t> msm_finish(const Xyzz<fq_t> *, uint32_t);
 template Jacobian<fq2_t> msm_finish(const Xyzz<fq2_t> *, uint32_t);
+template uint64_t msm_job_bytes<fq_t>(const MsmPlan &);
+template uint64_t msm_job_bytes<fq2_t>(const MsmPlan &);
 
 }  // namespace nkr

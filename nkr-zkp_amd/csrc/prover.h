@@ -64,10 +64,10 @@ private:
     MsmBases<fq_t> a_, b1_, l_, h_;
     MsmBases<fq2_t> b2_;
     MsmSchedule wsched_, hsched_;
-    MsmBatch<fq_t> batch_w_, batch_h_;
+    MsmBatch<fq_t> batch_g1_;            // ar, bs1, zk and H as one batch
     MsmBatch<fq2_t> batch_g2_;
-    hipStream_t stream2_ = nullptr;      // witness MSMs, concurrent with the NTT chain
-    hipEvent_t ev_start_ = nullptr, ev_side_done_ = nullptr;
+    hipStream_t stream2_ = nullptr;      // witness schedule and the G2 MSM, beside the NTT chain
+    hipEvent_t ev_start_ = nullptr, ev_wsched_ = nullptr, ev_side_done_ = nullptr;
     DeviceBuffer<fr_t> z_, pa_, pb_, pc_;
     DeviceBuffer<Xyzz<fq_t>> planes1_;
     DeviceBuffer<Xyzz<fq2_t>> planes2_;

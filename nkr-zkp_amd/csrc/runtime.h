@@ -120,11 +120,16 @@ public:
         }
         return ptr_;
     }
-    // Marks the end of the uploads from the buffer on `stream`.
+    // Marks the end of the transfers from or into the buffer on `stream`.
     void uploaded(hipStream_t stream)
     {
         if (!event_) NKR_HIP(hipEventCreateWithFlags(&event_, hipEventDisableTiming));
         NKR_HIP(hipEventRecord(event_, stream));
+    }
+    // Blocks until those transfers are done (a download is then readable).
+    void wait() const
+    {
+        if (event_) NKR_HIP(hipEventSynchronize(event_));
     }
     T *get() const { return ptr_; }
 

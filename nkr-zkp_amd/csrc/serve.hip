@@ -6,35 +6,12 @@
 
 namespace nkr {
 
-namespace {
-
-// Device bytes of one digit schedule for a plan (MsmSchedule buffers).
-uint64_t schedule_bytes(const MsmPlan &p)
-{
-    const uint64_t blocks = std::min<uint64_t>(1024, (p.n + 255) / 256);
-    return p.max_entries() * 4 + (uint64_t)p.B * 4 * 3 + blocks * p.B * 4 + 16;
-}
-
-// Partial-sum pool and per-bucket arrays of one MSM job (MsmBatch layout).
-template <class F>
-uint64_t job_bytes(const MsmPlan &p)
-{
-    uint64_t cap = p.max_entries() / p.piece + p.B + 1, cnt = cap, total = cap;
-    while (cnt > kMsmCompleteMax) {
-        cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
-        cap = cap / kMsmFoldIn + p.B + 1;
-        total += cap;
-    }
-    return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12) + p.c * sizeof(Xyzz<F>);
-}
-
-}  // namespace
-
 uint64_t Server::request_bytes(const Groth16Prover &g)
 {
     const uint64_t polys = (uint64_t)g.cols() * sizeof(fr_t) + 3ull * g.rows() * sizeof(fr_t);
-    const MsmPlan &w = g.witness_plan(), &h = g.h_plan();
-    return polys + schedule_bytes(w) + schedule_bytes(h) + 4 * job_bytes<fq_t>(w) + job_bytes<fq2_t>(w);
+    const MsmPlan w = batch_plan(g.witness_plan()), h = batch_plan(g.h_plan());
+    return polys + msm_schedule_bytes(w) + msm_schedule_bytes(h) + 3 * msm_job_bytes<fq_t>(w) +
+           msm_job_bytes<fq_t>(h) + msm_job_bytes<fq2_t>(w);
 }
 
 Server::Server(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes)

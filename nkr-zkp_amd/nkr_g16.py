@@ -82,6 +82,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
             "nkr_dev_copy": [vp, vp, u64, vp],
             "nkr_ntt_set_shape": [u32, u32, u32],
+            "nkr_msm_set_shape": [u32, u32],
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
