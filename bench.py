@@ -312,13 +312,14 @@ def main() -> int:
 
 
 def _msm_window(n: int) -> int:
-    """MsmPlan::choose: ~3/4 log2 n in [4, 15], moved to a neighbour when the
-    top window would keep fewer than 4 of the scalar's 254 bits."""
+    """MsmPlan::choose: ~3/4 log2 n in [4, 15] (16 from 2^20 terms up), moved
+    to a neighbour when the top window would keep fewer than 4 of the
+    scalar's 254 bits."""
     lg = max(0, (n - 1).bit_length())
-    c = max(4, min(15, (lg * 3 + 2) // 4))
+    c = 16 if lg >= 20 else max(4, min(15, (lg * 3 + 2) // 4))
     top = lambda cc: 254 - cc * ((255 + cc - 1) // cc - 1)
     if top(c) < 4:
-        if c < 15 and top(c + 1) >= 4:
+        if c < 16 and top(c + 1) >= 4:
             c += 1
         elif c > 4 and top(c - 1) >= 4:
             c -= 1

@@ -55,8 +55,9 @@ constexpr uint32_t kMaxFoldLevels = 12;
 // clamped to [piece_lo, piece_hi], so a schedule has about 2^target_log
 // pieces (default 2^19: every resident accumulation thread busy).
 constexpr uint32_t kMsmTargetPiecesLog = 19;
-// Largest bucket count (c <= 15): the bucket scans stage B counts in 64 KiB of LDS.
-constexpr uint32_t kMsmMaxBuckets = 1u << 14;
+// Largest bucket count (c <= 16): the histograms and bucket scans hold B
+// counts in one workgroup's LDS (128 KiB of gfx950's 160 KiB).
+constexpr uint32_t kMsmMaxBuckets = 1u << 15;
 
 // Process-wide MSM tuning knobs: the target piece count (2^target_log, 10 ..
 // 22) and the most lanes that complete one bucket (1 .. 16, a power of two).
@@ -191,8 +192,8 @@ private:
     uint32_t fold_levels_ = 0;
     std::vector<uint64_t> fold_cap_;  // largest partial count of any job at each fold level
     uint64_t thread_total_ = 0;
-    uint64_t complete_bound_ = 0;                      // completion threads at 16 lanes per bucket
-    DeviceBuffer<uint32_t> complete_base_, complete_lanes_;
+    uint64_t complete_bound_ = 0;  // completion lanes at most
+    DeviceBuffer<uint32_t> complete_base_, class_count_, class_list_;
     std::vector<uint64_t> thread_base_;
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;

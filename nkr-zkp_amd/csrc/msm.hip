@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 namespace nkr {
@@ -21,6 +22,8 @@ constexpr uint32_t kScanGroups = 16;
 // LDS per tree workgroup (two buffers of 2^h blocks); 64 KiB keeps two
 // workgroups per CU.
 constexpr size_t kTreeLdsBytes = 64 * 1024;
+// Completion lane classes (1, 2, 4, 8 lanes per bucket).
+constexpr uint32_t kCompleteClasses = 4;
 
 template <class F>
 struct JobDev {
@@ -38,6 +41,7 @@ struct JobDev {
     Xyzz<F> *tree[2];              // B each
     Xyzz<F> *planes;               // c
     uint32_t B, c;
+    uint32_t list_off;                    // this job's completion lists start at 4 * list_off
     uint32_t level_base[kMaxFoldLevels];  // where each fold level's partials start in `pool`
 };
 
@@ -323,10 +327,12 @@ __global__ __launch_bounds__(kSortThreads) void k_scatter_batch(const SortJob *_
 }
 
 template <class F>
-__global__ __launch_bounds__(256) void k_regions_init(const JobDev<F> *__restrict__ jobs)
+__global__ __launch_bounds__(256) void k_regions_init(const JobDev<F> *__restrict__ jobs,
+                                                      uint32_t *__restrict__ class_count)
 {
     const JobDev<F> &jb = jobs[blockIdx.y];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < kCompleteClasses) class_count[kCompleteClasses * blockIdx.y + k] = 0;
     if (k >= jb.B) return;
     jb.reg_start[k] = jb.piece_start[k];
     jb.reg_count[k] = jb.piece_start[k + 1] - jb.piece_start[k];
@@ -456,30 +462,60 @@ __device__ __forceinline__ Xyzz<F> shfl_xor_point(const Xyzz<F> &p, int mask)
     return r;
 }
 
-// Lanes per bucket for each job's completion and the job offsets of the
-// completion index space.  A bucket with m partials summed by L lanes costs
-// ceil(m / L) - 1 serial additions plus log2 L shuffle levels, and every lane
-// of a wave pays each shuffle level; L = nextpow2(m / 8) (1 .. 16) for the
-// job's largest bucket (m <= kMsmCompleteMax after folding) keeps the serial
-// part short without lanes that mostly add infinity.  Jobs are padded to
-// whole waves so that every wave completes buckets of one job with one L.
+// Completion classes: a bucket with m partials (m <= kMsmCompleteMax = 64
+// after folding) is summed by L = nextpow2(ceil(m / 8)) lanes (1, 2, 4 or 8:
+// at most 8 serial additions, then log2 L shuffle levels), so light buckets
+// do not drag idle lanes through a shuffle tree sized for the heaviest one.
+// Class c = log2 L; segment 4j + c lists job j's buckets of class c.
+__device__ __forceinline__ uint32_t complete_class(uint32_t m, uint32_t max_class)
+{
+    uint32_t c = 0;
+    while (c < max_class && (8u << c) < m) ++c;
+    return c;
+}
+
+// Buckets with no partial get S = infinity here; the others are appended to
+// their class list (one atomic per wave and class: a ballot ranks the lanes).
 template <class F>
-__global__ __launch_bounds__(1024) void k_complete_scan(const JobDev<F> *__restrict__ jobs, uint32_t J,
-                                                        uint32_t lanes_cap, uint32_t *__restrict__ base,
-                                                        uint32_t *__restrict__ lanes)
+__global__ __launch_bounds__(256) void k_classify(const JobDev<F> *__restrict__ jobs, uint32_t max_class,
+                                                  uint32_t *__restrict__ class_count,
+                                                  uint32_t *__restrict__ class_list)
+{
+    const uint32_t j = blockIdx.y;
+    const JobDev<F> &jb = jobs[j];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = k < jb.B;
+    const uint32_t m = in ? jb.reg_count[k] : 0u;
+    if (in && m == 0) jb.sums[k] = Xyzz<F>::infinity();
+    const uint32_t c = complete_class(m, max_class);
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t *list = class_list + (size_t)jb.list_off * kCompleteClasses;
+    for (uint32_t cc = 0; cc <= max_class; ++cc) {
+        const bool mine = in && m > 0 && c == cc;
+        const uint64_t mask = __ballot(mine);
+        if (mask == 0) continue;
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t first = 0;
+        if (lane == leader) first = atomicAdd(&class_count[kCompleteClasses * j + cc], (uint32_t)__popcll(mask));
+        first = (uint32_t)__shfl((int)first, (int)leader);
+        if (mine) list[(size_t)cc * jb.B + first + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = k;
+    }
+}
+
+// Lane offsets of the completion segments: segment s = 4j + c takes
+// count * 2^c lanes, padded to whole waves (every wave completes buckets of
+// one segment, in aligned groups of 2^c lanes).  One workgroup.
+__global__ __launch_bounds__(1024) void k_complete_scan(const uint32_t *__restrict__ class_count, uint32_t S,
+                                                        uint32_t *__restrict__ base)
 {
     __shared__ uint32_t part[1024];
-    auto job_lanes = [&](uint32_t j) -> uint32_t {
-        const uint32_t m = min(*jobs[j].piece_max, kMsmCompleteMax);
-        uint32_t L = 1;
-        while (8 * L < m && L < lanes_cap) L <<= 1;  // about 8 partials per lane
-        return L;
+    auto span = [&](uint32_t s) -> uint32_t {
+        return ((class_count[s] << (s % kCompleteClasses)) + 63u) & ~63u;
     };
-    auto job_span = [&](uint32_t j, uint32_t L) -> uint32_t { return (jobs[j].B * L + 63u) & ~63u; };
-    const uint32_t per = (J + blockDim.x - 1) / blockDim.x;
-    const uint32_t beg = min(J, threadIdx.x * per), end = min(J, beg + per);
+    const uint32_t per = (S + blockDim.x - 1) / blockDim.x;
+    const uint32_t beg = min(S, threadIdx.x * per), end = min(S, beg + per);
     uint32_t sum = 0;
-    for (uint32_t j = beg; j < end; ++j) sum += job_span(j, job_lanes(j));
+    for (uint32_t s = beg; s < end; ++s) sum += span(s);
     part[threadIdx.x] = sum;
     __syncthreads();
     for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
@@ -489,38 +525,40 @@ __global__ __launch_bounds__(1024) void k_complete_scan(const JobDev<F> *__restr
         __syncthreads();
     }
     uint32_t run = part[threadIdx.x] - sum;
-    for (uint32_t j = beg; j < end; ++j) {
-        const uint32_t L = job_lanes(j);
-        base[j] = run;
-        lanes[j] = L;
-        run += job_span(j, L);
+    for (uint32_t s = beg; s < end; ++s) {
+        base[s] = run;
+        run += span(s);
     }
-    if (threadIdx.x == blockDim.x - 1) base[J] = part[threadIdx.x];
+    if (threadIdx.x == blockDim.x - 1) base[S] = part[threadIdx.x];
 }
 
-// Bucket completion: L lanes per bucket stride over its partials, then a
-// shuffle tree of log2 L levels; grid-stride over all jobs' buckets.
+// Bucket completion: the 2^c lanes of a class-c bucket stride over its
+// partials, then a shuffle tree of c levels; grid-stride over all segments.
 template <class F>
 __global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ jobs,
-                                                  const uint32_t *__restrict__ base,
-                                                  const uint32_t *__restrict__ lanes, uint32_t J)
+                                                  const uint32_t *__restrict__ class_count,
+                                                  const uint32_t *__restrict__ class_list,
+                                                  const uint32_t *__restrict__ base, uint32_t S)
 {
-    const uint32_t total = base[J];
+    const uint32_t total = base[S];
 #pragma unroll 1
     for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < total; g += gridDim.x * blockDim.x) {
-        const uint32_t j = find_segment(base, J, g);
+        const uint32_t s = find_segment(base, S, g);
+        const uint32_t j = s / kCompleteClasses, c = s % kCompleteClasses;
         const JobDev<F> &jb = jobs[j];
-        const uint32_t L = lanes[j];
-        const uint32_t local = g - base[j];
-        const uint32_t k = local / L, lane = local & (L - 1);
+        const uint32_t local = g - base[s];
+        const uint32_t idx = local >> c, lane = local & ((1u << c) - 1u);
+        const bool live = idx < class_count[s];
+        const uint32_t k =
+            live ? class_list[(size_t)jb.list_off * kCompleteClasses + (size_t)c * jb.B + idx] : 0u;
         Xyzz<F> acc = Xyzz<F>::infinity();
-        if (k < jb.B) {
+        if (live) {
             const uint32_t start = jb.reg_start[k];
             const uint32_t cnt = jb.reg_count[k];
-            for (uint32_t i = lane; i < cnt; i += L) acc = xyzz_add(acc, jb.pool[start + i]);
+            for (uint32_t i = lane; i < cnt; i += (1u << c)) acc = xyzz_add(acc, jb.pool[start + i]);
         }
-        for (uint32_t off = L >> 1; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, (int)off));
-        if (k < jb.B && lane == 0) jb.sums[k] = acc;
+        for (uint32_t off = (1u << c) >> 1; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, (int)off));
+        if (live && lane == 0) jb.sums[k] = acc;
     }
 }
 
@@ -656,20 +694,21 @@ MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
     uint32_t c = c_override;
     if (c == 0) {
         const unsigned lg = ceil_log2(n);
-        // ~3/4 log2 n, clamped so the bucket histogram fits 64 KiB of LDS.
-        c = std::max(4u, std::min(15u, (lg * 3 + 2) / 4));
+        // ~3/4 log2 n; 16 bits from 2^20 terms up (16 windows instead of 17,
+        // its 128 KiB bucket histogram still fits one workgroup's LDS).
+        c = lg >= 20 ? 16u : std::max(4u, std::min(15u, (lg * 3 + 2) / 4));
         // Scalars have 254 bits; a top window with only a few of them sends a
         // whole window's entries to a handful of buckets (long serial chains),
         // so move to a neighbouring width whose top window keeps >= 4 bits.
         auto top_bits = [](uint32_t cc) { return 254 - cc * ((255 + cc - 1) / cc - 1); };
         if (top_bits(c) < 4) {
-            if (c < 15 && top_bits(c + 1) >= 4)
+            if (c < 16 && top_bits(c + 1) >= 4)
                 c += 1;
             else if (c > 4 && top_bits(c - 1) >= 4)
                 c -= 1;
         }
     }
-    NKR_REQUIRE(c >= 2 && c <= 15, "MSM window must be 2..15 bits");
+    NKR_REQUIRE(c >= 2 && c <= 16, "MSM window must be 2..16 bits");
     p.c = c;
     p.W = (255 + c - 1) / c;
     p.B = 1u << (c - 1);
@@ -715,8 +754,27 @@ void MsmBases<F>::prepare(const Jacobian<F> *jacobian, const uint32_t *perm, con
     NKR_LAUNCH_CHECK();
 }
 
+// The sort kernels hold B counts in dynamic LDS: up to 128 KiB for c = 16,
+// above the 64 KiB a launch gets without asking.
+static void allow_sort_lds()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const int bytes = (int)(kMsmMaxBuckets * sizeof(uint32_t));
+        NKR_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_hist),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        NKR_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        NKR_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_hist_batch),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        NKR_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter_batch),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    });
+}
+
 uint32_t MsmSchedule::prepare(const MsmPlan &plan)
 {
+    allow_sort_lds();
     NKR_REQUIRE(plan.B <= kMsmMaxBuckets, "MSM window too wide for the bucket scans");
     NKR_REQUIRE(plan.piece_lo >= 1 && plan.piece_lo <= plan.piece_hi, "MSM piece bounds");
     plan_ = plan;
@@ -884,15 +942,18 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.planes = jb.planes;
         d.B = jb.sched->plan().B;
         d.c = jb.sched->plan().c;
+        d.list_off = (uint32_t)b_off[j];
         for (uint32_t L = 0; L < kMaxFoldLevels; ++L) d.level_base[L] = level_base[j][L];
     }
     thread_total_ = thread_base_[J];
+    // completion lanes: at most 8 per bucket, plus each segment's padding
     complete_bound_ = 0;
-    for (uint32_t j = 0; j < J; ++j) complete_bound_ += ((uint64_t)host[j].B * 16 + 63) & ~uint64_t(63);
+    for (uint32_t j = 0; j < J; ++j) complete_bound_ += (uint64_t)host[j].B * 8 + kCompleteClasses * 64;
     NKR_REQUIRE(complete_bound_ < (1ull << 32) && thread_total_ < (1ull << 32),
                 "MSM batch too large for 32-bit work indices");
-    complete_base_.ensure(J + 1);
-    complete_lanes_.ensure(J);
+    complete_base_.ensure(kCompleteClasses * J + 1);
+    class_count_.ensure(kCompleteClasses * J);
+    class_list_.ensure(kCompleteClasses * b_off[J]);
     jobs_dev_.ensure(sizeof(JobDev<F>) * J);
     thread_base_dev_.ensure(J + 1);  // actual piece offsets, filled on the device per run
     NKR_HIP(hipMemcpyAsync(jobs_dev_.get(), host, sizeof(JobDev<F>) * J, hipMemcpyHostToDevice, stream));
@@ -913,7 +974,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     }
     const JobDev<F> *jd = reinterpret_cast<const JobDev<F> *>(jobs_dev_.get());
 
-    k_regions_init<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd);
+    k_regions_init<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, class_count_.get());
     NKR_LAUNCH_CHECK();
     if (timer) timer->begin(kStageAccumulate, stream);
     k_piece_scan<F><<<1, 1024, 0, stream>>>(jd, J, thread_base_dev_.get());
@@ -947,13 +1008,18 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
         k_fold_retarget<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
     }
-    k_complete_scan<F><<<1, 1024, 0, stream>>>(jd, J, msm_complete_lanes_cap(), complete_base_.get(),
-                                               complete_lanes_.get());
+    uint32_t max_class = 0;
+    while (max_class + 1 < kCompleteClasses && (2u << max_class) <= msm_complete_lanes_cap()) ++max_class;
+    k_classify<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, max_class, class_count_.get(),
+                                                                   class_list_.get());
+    NKR_LAUNCH_CHECK();
+    const uint32_t S = kCompleteClasses * J;
+    k_complete_scan<<<1, 1024, 0, stream>>>(class_count_.get(), S, complete_base_.get());
     NKR_LAUNCH_CHECK();
     {
         const uint64_t blocks = std::min<uint64_t>(div_up(complete_bound_, 256), resident_blocks<k_complete<F>>());
         k_complete<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(
-            jd, complete_base_.get(), complete_lanes_.get(), J);
+            jd, class_count_.get(), class_list_.get(), complete_base_.get(), S);
     }
     NKR_LAUNCH_CHECK();
     // bit-plane tree: as many levels per launch as two LDS buffers of 2^h

@@ -67,10 +67,12 @@ private:
     MsmBatch<fq_t> batch_g1_;            // ar, bs1, zk and H as one batch
     MsmBatch<fq2_t> batch_g2_;
     hipStream_t stream2_ = nullptr;      // witness schedule and the G2 MSM, beside the NTT chain
-    hipEvent_t ev_start_ = nullptr, ev_wsched_ = nullptr, ev_side_done_ = nullptr;
+    hipEvent_t ev_start_ = nullptr, ev_wsched_ = nullptr;
     DeviceBuffer<fr_t> z_, pa_, pb_, pc_;
     DeviceBuffer<Xyzz<fq_t>> planes1_;
     DeviceBuffer<Xyzz<fq2_t>> planes2_;
+    PinnedStaging<Xyzz<fq_t>> planes1_host_;
+    PinnedStaging<Xyzz<fq2_t>> planes2_host_;
     void *pinned_witness_ = nullptr;
 
     DeviceBuffer<fr_t *> poly_table_;  // {pa, pb, pc} for the batched transforms
@@ -87,6 +89,9 @@ private:
     BlindPre blind_pre(const fr_t &r, const fr_t &s) const;
     void finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
                 void *raw_out) const;
+    // The two halves of finish: bs2 (and B) from the G2 planes, the rest from the G1 planes.
+    void finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre, void *raw_out) const;
+    void finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out) const;
     friend class BatchProver;
 
     // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)

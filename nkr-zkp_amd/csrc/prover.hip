@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <future>
 
 #include "setup.h"
 
@@ -72,7 +73,6 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
     NKR_HIP(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
     NKR_HIP(hipEventCreateWithFlags(&ev_wsched_, hipEventDisableTiming));
-    NKR_HIP(hipEventCreateWithFlags(&ev_side_done_, hipEventDisableTiming));
 
     const uint32_t *rps[3] = {c.a_row_ptr, c.b_row_ptr, c.c_row_ptr};
     const uint32_t *cis[3] = {c.a_col_idx, c.b_col_idx, c.c_col_idx};
@@ -157,7 +157,6 @@ Groth16Prover::~Groth16Prover()
     }
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (ev_wsched_) (void)hipEventDestroy(ev_wsched_);
-    if (ev_side_done_) (void)hipEventDestroy(ev_side_done_);
     if (pinned_witness_) (void)hipHostFree(pinned_witness_);
 }
 
@@ -263,30 +262,52 @@ Groth16Prover::BlindPre Groth16Prover::blind_pre(const fr_t &r, const fr_t &s) c
 void Groth16Prover::finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
                            void *raw_out) const
 {
+    finish_g2(p2, blind, pre, raw_out);
+    finish_g1(p1, blind, pre, raw_out);
+}
+
+void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
+                              void *raw_out) const
+{
+    const g2_jac bs2 = msm_finish(p2, wplan_.c);
+    std::memcpy(static_cast<uint8_t *>(raw_out) + 384, &bs2, 192);
+    if (!blind) return;
+    const g2_jac B = xyzz_to_jacobian_affine(xyzz_add(pre->B0, jacobian_to_xyzz(bs2)));
+    std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 96, &B, 192);
+}
+
+void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre,
+                              void *raw_out) const
+{
     const uint32_t cw = wplan_.c;
     uint8_t *out = static_cast<uint8_t *>(raw_out);
+    // zk and H on a helper thread while this one does ar, bs1 and the r,s
+    // products that need them
+    g1_jac zk, qz;
+    auto other = std::async(std::launch::async, [&] {
+        zk = msm_finish(p1 + 2 * cw, cw);
+        qz = msm_finish(p1 + 3 * cw, hplan_.c);
+    });
     const g1_jac ar = msm_finish(p1 + 0 * cw, cw);
     const g1_jac bs1 = msm_finish(p1 + 1 * cw, cw);
-    const g1_jac zk = msm_finish(p1 + 2 * cw, cw);
-    const g1_jac qz = msm_finish(p1 + 3 * cw, hplan_.c);
-    const g2_jac bs2 = msm_finish(p2, cw);
+    g1_xyzz rs_part;
+    if (blind) {
+        const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
+        const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(pre->A0, xar));
+        std::memcpy(static_cast<uint8_t *>(blind->proof_out), &A, 96);
+        rs_part = straus2(xar, blind->s, xbs1, blind->r);
+    }
+    other.get();
     std::memcpy(out + 0, &ar, 96);
     std::memcpy(out + 96, &bs1, 96);
     std::memcpy(out + 192, &zk, 96);
     std::memcpy(out + 288, &qz, 96);
-    std::memcpy(out + 384, &bs2, 192);
     if (!blind) return;
-    const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
-    const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(pre->A0, xar));
-    const g2_jac B = xyzz_to_jacobian_affine(xyzz_add(pre->B0, jacobian_to_xyzz(bs2)));
     g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
     C = xyzz_add(C, pre->C0);
-    C = xyzz_add(C, straus2(xar, blind->s, xbs1, blind->r));
+    C = xyzz_add(C, rs_part);
     const g1_jac Cj = xyzz_to_jacobian_affine(C);
-    uint8_t *o = static_cast<uint8_t *>(blind->proof_out);
-    std::memcpy(o, &A, 96);
-    std::memcpy(o + 96, &B, 192);
-    std::memcpy(o + 288, &Cj, 96);
+    std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 288, &Cj, 96);
 }
 
 void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind)
@@ -333,7 +354,9 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     wsched_.build(z, wplan_, stream2_, t);
     NKR_HIP(hipEventRecord(ev_wsched_, stream2_));
     batch_g2_.run({MsmJob<fq2_t>{&b2_, &wsched_, planes2_.get()}}, stream2_, t);
-    NKR_HIP(hipEventRecord(ev_side_done_, stream2_));
+    Xyzz<fq2_t> *h2 = planes2_host_.reuse(cw);
+    NKR_HIP(hipMemcpyAsync(h2, planes2_.get(), cw * sizeof(Xyzz<fq2_t>), hipMemcpyDeviceToHost, stream2_));
+    planes2_host_.uploaded(stream2_);
 
     // The G1 MSMs ar, bs1, zk (witness) and H as one batch on the main
     // stream: the witness pieces fill the H accumulation launch, and the
@@ -345,23 +368,30 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
                    MsmJob<fq_t>{&l_, &wsched_, planes1_.get() + 2 * cw},
                    MsmJob<fq_t>{&h_, &hsched_, planes1_.get() + 3 * cw}},
                   s, t);
-    NKR_HIP(hipStreamWaitEvent(s, ev_side_done_, 0));
+    const size_t n1 = 3 * cw + hplan_.c;
+    Xyzz<fq_t> *h1 = planes1_host_.reuse(n1);
+    NKR_HIP(hipMemcpyAsync(h1, planes1_.get(), n1 * sizeof(Xyzz<fq_t>), hipMemcpyDeviceToHost, s));
+    planes1_host_.uploaded(s);
 
-    std::vector<Xyzz<fq_t>> p1(3 * cw + hplan_.c);
-    std::vector<Xyzz<fq2_t>> p2(cw);
-    NKR_HIP(hipMemcpyAsync(p1.data(), planes1_.get(), p1.size() * sizeof(p1[0]), hipMemcpyDeviceToHost, s));
-    NKR_HIP(hipMemcpyAsync(p2.data(), planes2_.get(), p2.size() * sizeof(p2[0]), hipMemcpyDeviceToHost, s));
-
-    // blinding terms that need only r and s, while the GPU works
+    // blinding terms that need only r and s, then the G2 tail as soon as the
+    // side stream is done, both while the GPU works on the G1 batch
     BlindPre pre;
     if (blind) pre = blind_pre(blind->r, blind->s);
-    NKR_HIP(hipStreamSynchronize(s));
-    const auto host0 = std::chrono::steady_clock::now();
-    finish(p1.data(), p2.data(), blind, blind ? &pre : nullptr, raw_out);
+    double host_ms = 0;
+    planes2_host_.wait();
+    auto host0 = std::chrono::steady_clock::now();
+    finish_g2(h2, blind, blind ? &pre : nullptr, raw_out);
+    host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
+    planes1_host_.wait();
+    host0 = std::chrono::steady_clock::now();
+    finish_g1(h1, blind, blind ? &pre : nullptr, raw_out);
+    host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
 
     if (t) {
+        NKR_HIP(hipStreamSynchronize(s));
+        NKR_HIP(hipStreamSynchronize(stream2_));
         t->collect(stage_ms_);
-        stage_ms_[6] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
+        stage_ms_[6] += host_ms;
         counters_[0] += 1;
         counters_[1] += 2;  // k_accumulate launches: G1 batch (witness and H), G2
         uint32_t tw = 0, th = 0;
