@@ -14,13 +14,16 @@ constexpr unsigned kMaxThreads = 512;
 
 // Workgroup shape of the pass kernels: 2^lds_log elements of LDS per
 // workgroup and 2^radix_log elements per thread per register round
-// (threads = 2^(lds_log - radix_log)).  Chosen by measurement on MI355X.
+// (threads = 2^(lds_log - radix_log)).  Chosen by measurement on MI355X
+// inside whole proofs, where the MSM side stream shares the CUs: radix-4
+// rounds need about half the registers of radix-8 ones, which leaves room
+// for the other stream's waves.
 unsigned g_lds_log = 11;
-unsigned g_radix_log = 3;
+unsigned g_radix_log = 2;
 // Pass split for transforms larger than one LDS tile: 0 balances the bits
 // between passes, 1 gives the unit-stride last pass a full tile and keeps
 // the strided passes narrow enough to gather several adjacent columns.
-unsigned g_split_policy = 0;
+unsigned g_split_policy = 1;
 
 struct PassArgs {
     fr_t *data;
