@@ -76,9 +76,10 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
 
     // witnesses -> pinned staging -> slot buffers
     size_t wbytes = 0, pbytes = 0;
-    uint32_t max_rows = 0;
+    uint32_t max_rows = 0, max_long = 0;
     for (const BatchItem &it : items) {
         const Groth16Prover &g = *it.ctx;
+        for (int m = 0; m < 3; ++m) max_long = std::max(max_long, g.views_[m].n_long);
         wbytes += (size_t)g.cols_ * sizeof(fr_t);
         pbytes += (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>) + g.wplan_.c * sizeof(Xyzz<fq2_t>);
         max_rows = std::max(max_rows, g.rows_);
@@ -152,7 +153,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     // fills the device, so a second stream would only make the stages wait
     // for each other's workgroup slots.
     // A*z, B*z, C*z, then the quotient chain per size class
-    spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, stream_);
+    spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, max_long, stream_);
     for (const ClassTables &ct : ctabs) {
         fr_t *const *all3 = reinterpret_cast<fr_t *const *>(tables_.get() + ct.all3);
         fr_t *const *a_only = reinterpret_cast<fr_t *const *>(tables_.get() + ct.a_only);

@@ -543,6 +543,9 @@ int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32
         }
         if (cols) NKR_HIP(hipMemcpy(dz.get(), z, cols * sizeof(fr_t), hipMemcpyHostToDevice));
         CsrView m{rp.get(), ci.get(), v.get(), (uint32_t)rows, (uint32_t)nnz};
+        DeviceBuffer<uint32_t> lr;
+        m.n_long = spmv_long_rows(row_ptr, (uint32_t)rows, lr);
+        m.long_rows = lr.get();
         spmv(m, dz.get(), dy.get(), nullptr);
         if (rows) NKR_HIP(hipMemcpy(y, dy.get(), rows * sizeof(fr_t), hipMemcpyDeviceToHost));
     });

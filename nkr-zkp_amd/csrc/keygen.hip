@@ -169,7 +169,7 @@ void groth16_keygen(const nkr_g16_circuit &c, const Trapdoors &td, int device, c
     const uint32_t *cis[3] = {c.a_col_idx, c.b_col_idx, c.c_col_idx};
     const void *vs[3] = {c.a_values, c.b_values, c.c_values};
     const uint64_t nnz[3] = {c.a_nnz, c.b_nnz, c.c_nnz};
-    DeviceBuffer<uint32_t> trp[3], tci[3];
+    DeviceBuffer<uint32_t> trp[3], tci[3], tlong[3];
     DeviceBuffer<fr_t> tval[3];
     CsrView views[3];
     for (int m = 0; m < 3; ++m) {
@@ -180,6 +180,8 @@ void groth16_keygen(const nkr_g16_circuit &c, const Trapdoors &td, int device, c
         // the host vectors die at the end of this iteration
         NKR_HIP(hipStreamSynchronize(s));
         views[m] = CsrView{trp[m].get(), tci[m].get(), tval[m].get(), cols, (uint32_t)nnz[m]};
+        views[m].n_long = spmv_long_rows(t.row_ptr.data(), cols, tlong[m]);
+        views[m].long_rows = tlong[m].get();
     }
     DeviceBuffer<fr_t> at(cols), bt(cols), ct(cols), l(cols), ic(pub + 1), h(n);
     spmv3(views[0], views[1], views[2], lag.get(), at.get(), bt.get(), ct.get(), s);

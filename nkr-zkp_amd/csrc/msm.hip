@@ -19,9 +19,14 @@ constexpr uint32_t kSortMinPerBlock = 256;
 constexpr uint32_t kSortThreads = 1024;
 // Column scan tile: 64 buckets x kScanGroups row groups per workgroup.
 constexpr uint32_t kScanGroups = 16;
-// LDS per tree workgroup (two buffers of 2^h blocks); 64 KiB keeps two
-// workgroups per CU.
-constexpr size_t kTreeLdsBytes = 64 * 1024;
+// LDS per tree workgroup (two buffers of 2^h blocks): 64 KiB for G1 (two
+// workgroups per CU); 32 KiB for G2, whose trees run beside the main
+// stream's transforms and histograms and must leave them room on the CU.
+template <class F>
+constexpr size_t tree_lds_bytes()
+{
+    return std::is_same<F, fq2_t>::value ? 32 * 1024 : 64 * 1024;
+}
 // Completion lane classes (1, 2, 4, 8 lanes per bucket).
 constexpr uint32_t kCompleteClasses = 4;
 
@@ -386,7 +391,19 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
         const Affine<F> *__restrict__ bases = jb.bases;
         const uint32_t *__restrict__ sorted = jb.sorted;
         Xyzz<F> acc = Xyzz<F>::infinity();
-        if (beg < end) {
+        if (std::is_same<F, fq2_t>::value) {
+            // G2: no prefetch; the second base's 32 registers would push the
+            // addition's working set past the register file of two waves.
+#pragma unroll 1
+            for (uint32_t e = beg; e < end; ++e) {
+                const uint32_t v = sorted[e];
+                const Affine<F> P = bases[v & kEntryMask];
+                if (!affine_is_inf(P)) {
+                    const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
+                    acc = xyzz_add_affine(acc, P.x, y);
+                }
+            }
+        } else if (beg < end) {
             // Software pipeline: the gather of entry e + 1's base is issued
             // before entry e's addition, so its latency hides behind the
             // field arithmetic.
@@ -1026,7 +1043,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     // level-t blocks allow
     for (uint32_t t = 0, src = 0; t + 1 <= cmax - 1; src ^= 1u) {
         uint32_t h = 1;
-        while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(Xyzz<F>) <= kTreeLdsBytes) ++h;
+        while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(Xyzz<F>) <= tree_lds_bytes<F>()) ++h;
         const size_t lds = 2 * (size_t(1) << h) * (t + 1) * sizeof(Xyzz<F>);
         k_tree_lds<F><<<dim3(Bmax >> (t + h), J), 256, lds, stream>>>(jd, t, h, src);
         NKR_LAUNCH_CHECK();

@@ -56,7 +56,7 @@ private:
     int device_;
     hipStream_t stream_ = nullptr;
     uint32_t rows_, cols_, log_n_;
-    DeviceBuffer<uint32_t> row_ptr_[3], col_idx_[3];
+    DeviceBuffer<uint32_t> row_ptr_[3], col_idx_[3], long_rows_[3];
     DeviceBuffer<fr_t> values_[3];
     CsrView views_[3];
     std::unique_ptr<NttDomain> domain_;

@@ -84,10 +84,15 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
         upload(col_idx_[m], cis[m], nnz[m], stream_);
         upload(values_[m], vs[m], nnz[m], stream_);
         views_[m] = CsrView{row_ptr_[m].get(), col_idx_[m].get(), values_[m].get(), rows_, (uint32_t)nnz[m]};
+        views_[m].n_long = spmv_long_rows(rps[m], rows_, long_rows_[m]);
+        views_[m].long_rows = long_rows_[m].get();
     }
 
     domain_ = std::make_unique<NttDomain>(log_n_, stream_);
+    // The witness schedule is sorted on the side stream beside the transforms:
+    // at most 15-bit windows, whose 64 KiB histograms share a CU with them.
     wplan_ = MsmPlan::choose(cols_);
+    if (wplan_.c > 15) wplan_ = MsmPlan::choose(cols_, 15);
     hplan_ = MsmPlan::choose(rows_);
 
     {

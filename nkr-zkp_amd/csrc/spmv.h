@@ -14,7 +14,18 @@ struct CsrView {
     const fr_t *values;
     uint32_t rows;
     uint32_t nnz;
+    // Rows with more than kSpmvShortRow non-zeros, each summed by one wave
+    // (64 lanes and a shuffle reduction) instead of one thread; without the
+    // list every row is summed by one thread.
+    const uint32_t *long_rows = nullptr;
+    uint32_t n_long = 0;
 };
+
+constexpr uint32_t kSpmvShortRow = 16;
+
+// Uploads the list of rows of a host CSR offset array with more than
+// kSpmvShortRow non-zeros into `out` (synchronously) and returns its length.
+uint32_t spmv_long_rows(const uint32_t *row_ptr_host, uint32_t rows, DeviceBuffer<uint32_t> &out);
 
 void spmv3(const CsrView &a, const CsrView &b, const CsrView &c, const fr_t *z, fr_t *ya, fr_t *yb, fr_t *yc,
            hipStream_t stream);
@@ -26,6 +37,6 @@ struct SpmvJob {
     const fr_t *z;
     fr_t *y[3];
 };
-void spmv3_batch(const SpmvJob *jobs_dev, uint32_t count, uint32_t max_rows, hipStream_t stream);
+void spmv3_batch(const SpmvJob *jobs_dev, uint32_t count, uint32_t max_rows, uint32_t max_long, hipStream_t stream);
 
 }  // namespace nkr

@@ -112,6 +112,24 @@ def test_spmv_random_large():
     assert np.array_equal(got, O.spmv(rows, cols, row_ptr, col_idx, vals, z))
 
 
+def test_spmv_long_rows():
+    # rows above 16 non-zeros go to one wave each: mix them with short ones,
+    # the 16/17 boundary, empty rows and rows longer than a wave
+    rows, cols = 4096, 3000
+    rng = np.random.default_rng(11)
+    per_row = rng.integers(0, 5, rows)
+    per_row[::97] = rng.integers(17, 300, per_row[::97].size)
+    per_row[5], per_row[6], per_row[7], per_row[8] = 16, 17, 0, 64
+    per_row[9], per_row[10] = 65, 1000
+    row_ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    nnz = int(row_ptr[-1])
+    col_idx = rng.integers(0, cols, nnz).astype(np.uint32)
+    vals = rand_fr(403, nnz)
+    z = rand_fr(404, cols)
+    got = N.spmv(rows, cols, row_ptr, col_idx, vals, z)
+    assert np.array_equal(got, O.spmv(rows, cols, row_ptr, col_idx, vals, z))
+
+
 # ------------------------------------------------------------------ MSM
 def test_fixed_base_mul_matches_oracle():
     k = rand_fr(501, 300)
