@@ -107,7 +107,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     std::map<uint32_t, std::vector<size_t>> classes;
     for (size_t i = 0; i < K; ++i) classes[items[i].ctx->log_n_].push_back(i);
     size_t tbytes = align16(K * sizeof(SpmvJob));
-    for (const auto &cl : classes) tbytes += align16(cl.second.size() * 8 * sizeof(fr_t *));
+    for (const auto &cl : classes) tbytes += align16(cl.second.size() * 7 * sizeof(fr_t *));
     uint8_t *tst = table_stage_.reuse(tbytes);
     tables_.ensure(tbytes);
     SpmvJob *jobs = reinterpret_cast<SpmvJob *>(tst);
@@ -123,28 +123,26 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     struct ClassTables {
         const NttDomain *domain;
         uint32_t count;
-        size_t all3, a_only, quot;  // byte offsets into tables_
+        size_t all3, quot;  // byte offsets into tables_
     };
     std::vector<ClassTables> ctabs;
     off = align16(K * sizeof(SpmvJob));
     for (const auto &cl : classes) {
         const size_t k = cl.second.size();
         fr_t **p = reinterpret_cast<fr_t **>(tst + off);
-        ClassTables ct{items[cl.second[0]].ctx->domain_.get(), (uint32_t)k, off, off + 3 * k * sizeof(fr_t *),
-                       off + 4 * k * sizeof(fr_t *)};
+        ClassTables ct{items[cl.second[0]].ctx->domain_.get(), (uint32_t)k, off, off + 3 * k * sizeof(fr_t *)};
         for (size_t q = 0; q < k; ++q) {
             Slot &sl = *slots_[cl.second[q]];
             p[3 * q + 0] = sl.pa.get();
             p[3 * q + 1] = sl.pb.get();
             p[3 * q + 2] = sl.pc.get();
-            p[3 * k + q] = sl.pa.get();
-            p[4 * k + 4 * q + 0] = sl.pa.get();
-            p[4 * k + 4 * q + 1] = sl.pb.get();
-            p[4 * k + 4 * q + 2] = sl.pc.get();
-            p[4 * k + 4 * q + 3] = sl.pa.get();
+            p[3 * k + 4 * q + 0] = sl.pa.get();
+            p[3 * k + 4 * q + 1] = sl.pb.get();
+            p[3 * k + 4 * q + 2] = sl.pc.get();
+            p[3 * k + 4 * q + 3] = sl.pa.get();
         }
         ctabs.push_back(ct);
-        off += align16(k * 8 * sizeof(fr_t *));
+        off += align16(k * 7 * sizeof(fr_t *));
     }
     NKR_HIP(hipMemcpyAsync(tables_.get(), tst, tbytes, hipMemcpyHostToDevice, stream_));
     table_stage_.uploaded(stream_);
@@ -156,12 +154,10 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, max_long, stream_);
     for (const ClassTables &ct : ctabs) {
         fr_t *const *all3 = reinterpret_cast<fr_t *const *>(tables_.get() + ct.all3);
-        fr_t *const *a_only = reinterpret_cast<fr_t *const *>(tables_.get() + ct.a_only);
         fr_t *const *quot = reinterpret_cast<fr_t *const *>(tables_.get() + ct.quot);
         ct.domain->dif_batch(all3, 3 * ct.count, true, nullptr, nullptr, nullptr, stream_);
         ct.domain->dit_batch(all3, 3 * ct.count, false, ct.domain->coset_rev_scaled(), nullptr, stream_);
         ct.domain->quotient_batch(quot, ct.count, stream_);
-        ct.domain->dif_batch(a_only, ct.count, true, nullptr, ct.domain->icoset_rev_scaled(), nullptr, stream_);
     }
     // digit schedules of every witness and every H, then one G1 batch over
     // all four G1 MSMs of every proof and one G2 batch

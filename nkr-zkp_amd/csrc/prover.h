@@ -3,11 +3,12 @@
 // 270-335) and of CpuBackend::prove (backends/cpu/src/backend.cpp:156-201).
 //
 // prepare (constructor): CSR matrices and proving key to HBM, key points
-// normalised and window-expanded once (MsmBases), NTT tables built.
+// normalised and window-expanded once (MsmBases), NTT tables built, and the
+// H query moved to the basis of coset evaluations (hquery.h).
 // prove: witness H2D -> A*z, B*z, C*z -> iNTT x3 -> coset NTT x3 -> (ab - c)/Z
-// -> coset iNTT (H left in bit-reversed order; the H-query bases are stored
-// in the same order) -> one signed-digit sort of the witness shared by the
-// ar / bs1 / zk / bs2 MSMs -> one for H -> bit planes D2H -> host finish.
+// on the coset, which the H MSM takes as its scalars (no seventh transform)
+// -> one signed-digit sort of the witness shared by the ar / bs1 / zk / bs2
+// MSMs -> one for H -> bit planes D2H -> host finish.
 #pragma once
 
 #include <memory>

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <future>
 
+#include "hquery.h"
 #include "setup.h"
 
 namespace nkr {
@@ -105,13 +106,17 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
             g1_sets[k]->prepare(tmp1.get(), nullptr, wplan_, stream_);
             NKR_HIP(hipStreamSynchronize(stream_));
         }
-        // H-query in bit-reversed order, matching the order H leaves the chain in.
+        // H query in the basis of the quotient's coset evaluations (the prove
+        // chain then skips its last transform), which the group transform
+        // leaves in bit-reversed order: MSM term i uses position bitrev(i).
         const std::vector<uint32_t> perm = bitrev_permutation(log_n_);
         DeviceBuffer<uint32_t> dperm;
         upload(dperm, perm.data(), perm.size(), stream_);
         NKR_HIP(hipMemcpyAsync(tmp1.get(), pk.h_query, (size_t)rows_ * sizeof(g1_jac), hipMemcpyHostToDevice,
                                stream_));
-        h_.prepare(tmp1.get(), dperm.get(), hplan_, stream_);
+        DeviceBuffer<g1_jac> hq(rows_);
+        h_query_coset_basis(tmp1.get(), *domain_, hq.get(), stream_);
+        h_.prepare(hq.get(), dperm.get(), hplan_, stream_);
         NKR_HIP(hipStreamSynchronize(stream_));
     }
     {
@@ -342,15 +347,15 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
         t->begin(2, s);
     }
     domain_->quotient(pa_.get(), pb_.get(), pc_.get(), pa_.get(), s);
-    if (t) {
-        t->end(2, s);
-        t->begin(1, s);
-    }
-    domain_->dif(pa_.get(), true, nullptr, domain_->icoset_rev_scaled(), nullptr, s);
-    if (t) t->end(1, s);
+    if (t) t->end(2, s);
+    // pa now holds the quotient's coset evaluations q, which the H MSM takes
+    // directly (its bases are in the coset-evaluation basis).  The
+    // coefficients h = icosetFFT(q) are formed only when asked for.
     if (h) {
-        domain_->bitrev(pa_.get(), pb_.get(), s);
-        NKR_HIP(hipMemcpyAsync(h, pb_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
+        NKR_HIP(hipMemcpyAsync(pb_.get(), pa_.get(), poly_bytes, hipMemcpyDeviceToDevice, s));
+        domain_->dif(pb_.get(), true, nullptr, domain_->icoset_rev_scaled(), nullptr, s);
+        domain_->bitrev(pb_.get(), pc_.get(), s);
+        NKR_HIP(hipMemcpyAsync(h, pc_.get(), poly_bytes, hipMemcpyDeviceToHost, s));
     }
 
     // The witness schedule and the G2 MSM (bs2) depend only on z: they run on
