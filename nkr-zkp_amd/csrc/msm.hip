@@ -1040,12 +1040,16 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     }
     NKR_LAUNCH_CHECK();
     // bit-plane tree: as many levels per launch as two LDS buffers of 2^h
-    // level-t blocks allow
+    // level-t blocks allow; a workgroup has as many lanes as its widest
+    // level has merges (whole waves, at most 256), so no idle wave holds a
+    // SIMD's registers
     for (uint32_t t = 0, src = 0; t + 1 <= cmax - 1; src ^= 1u) {
         uint32_t h = 1;
         while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(Xyzz<F>) <= tree_lds_bytes<F>()) ++h;
         const size_t lds = 2 * (size_t(1) << h) * (t + 1) * sizeof(Xyzz<F>);
-        k_tree_lds<F><<<dim3(Bmax >> (t + h), J), 256, lds, stream>>>(jd, t, h, src);
+        const uint32_t widest = (1u << (h - 1)) * (t + 2);
+        const uint32_t lanes = std::min<uint32_t>(256, (uint32_t)div_up(widest, 64) * 64);
+        k_tree_lds<F><<<dim3(Bmax >> (t + h), J), lanes, lds, stream>>>(jd, t, h, src);
         NKR_LAUNCH_CHECK();
         t += h;
     }
