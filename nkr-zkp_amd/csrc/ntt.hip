@@ -52,13 +52,44 @@ __device__ __forceinline__ fr_t tw_pow(const PassArgs &a, uint32_t e)
     return fp_mul(a.tw_hi[hi], a.tw_lo[lo]);
 }
 
+// LDS tile of n field elements: two planes of 16-byte halves (element i's
+// low half in plane 0, high half in plane 1) so that every access is one
+// 16-byte LDS instruction per half, and within each aligned run of 64
+// elements the 16-element rows are XOR-permuted (row r by 5 * (r mod 4)),
+// which spreads the strided accesses of every radix-4 round over distinct
+// bank groups.
+__device__ __forceinline__ uint32_t lds_slot(uint32_t i) { return i ^ (5u * ((i >> 4) & 3u)); }
+
+__device__ __forceinline__ fr_t lds_get(const uint4 *tile, uint32_t n, uint32_t i)
+{
+    const uint32_t k = lds_slot(i);
+    const uint4 a = tile[k], b = tile[n + k];
+    fr_t r;
+    r.l[0] = a.x;
+    r.l[1] = a.y;
+    r.l[2] = a.z;
+    r.l[3] = a.w;
+    r.l[4] = b.x;
+    r.l[5] = b.y;
+    r.l[6] = b.z;
+    r.l[7] = b.w;
+    return r;
+}
+
+__device__ __forceinline__ void lds_put(uint4 *tile, uint32_t n, uint32_t i, const fr_t &v)
+{
+    const uint32_t k = lds_slot(i);
+    tile[k] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
+    tile[n + k] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+}
+
 // One round of R consecutive radix-2 stages (t .. t+R-1) on groups of 2^R
 // elements held in registers.  A group fixes every index bit except the R
 // bits [np-t-R, np-t) that the round's butterflies pair.  The twiddle of the
 // pair (u, u + 2^(R-1-k)) at stage s = t + k is root^((j mod 2^(np-1-s)) * 2^s)
 // for its lower index j, as in the radix-2 DIF recursion.
 template <bool DIT, int R>
-__device__ __forceinline__ void ntt_round(fr_t *smem, const PassArgs &a, uint32_t t, uint32_t cols,
+__device__ __forceinline__ void ntt_round(uint4 *smem, uint32_t n, const PassArgs &a, uint32_t t, uint32_t cols,
                                           uint32_t root_shift)
 {
     constexpr uint32_t G = 1u << R;
@@ -71,7 +102,7 @@ __device__ __forceinline__ void ntt_round(fr_t *smem, const PassArgs &a, uint32_
         const uint32_t j0 = ((q >> low) << (a.np - t)) | lo_part;
         fr_t x[G];
 #pragma unroll
-        for (uint32_t u = 0; u < G; ++u) x[u] = smem[(j0 + (u << low)) * cols + c];
+        for (uint32_t u = 0; u < G; ++u) x[u] = lds_get(smem, n, (j0 + (u << low)) * cols + c);
 #pragma unroll
         for (int kk = 0; kk < R; ++kk) {
             const int k = DIT ? (R - 1 - kk) : kk;
@@ -94,14 +125,14 @@ __device__ __forceinline__ void ntt_round(fr_t *smem, const PassArgs &a, uint32_
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < G; ++u) smem[(j0 + (u << low)) * cols + c] = x[u];
+        for (uint32_t u = 0; u < G; ++u) lds_put(smem, n, (j0 + (u << low)) * cols + c, x[u]);
     }
 }
 
 template <bool DIT, int RMAX>
 __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
 {
-    extern __shared__ fr_t smem[];
+    extern __shared__ uint4 smem[];
     if (a.batch) a.data = a.batch[blockIdx.y];
     const uint32_t G = 1u << a.np;
     const uint32_t cols = 1u << a.log_cols;
@@ -129,7 +160,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
             if (e) v = fp_mul(v, tw_pow(a, e));
         }
-        smem[j * cols + c] = v;
+        lds_put(smem, total, j * cols + c, v);
     }
     __syncthreads();
 
@@ -142,11 +173,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
         const uint32_t r = min((uint32_t)RMAX, a.np - done);
         const uint32_t t = DIT ? (a.np - done - r) : done;
         if (r == 3)
-            ntt_round<DIT, 3>(smem, a, t, cols, root_shift);
+            ntt_round<DIT, 3>(smem, total, a, t, cols, root_shift);
         else if (r == 2)
-            ntt_round<DIT, 2>(smem, a, t, cols, root_shift);
+            ntt_round<DIT, 2>(smem, total, a, t, cols, root_shift);
         else
-            ntt_round<DIT, 1>(smem, a, t, cols, root_shift);
+            ntt_round<DIT, 1>(smem, total, a, t, cols, root_shift);
         __syncthreads();
         done += r;
     }
@@ -164,7 +195,7 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
         const uint64_t hi = g >> a.log_L;
         const uint32_t lo = (uint32_t)(g & (L - 1));
         const uint64_t pos = (hi << (a.log_n - a.S)) + ((uint64_t)j << a.log_L) + lo;
-        fr_t v = smem[j * cols + c];
+        fr_t v = lds_get(smem, total, j * cols + c);
         if (!DIT && a.twiddle) {
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
             if (e) v = fp_mul(v, tw_pow(a, e));
