@@ -222,7 +222,11 @@ NKR_DEV void mac96(uint64_t &acc, uint32_t &top, uint32_t x, uint32_t y)
 // Montgomery product by product scanning (column-wise, "FIPS" order): column
 // k accumulates a_j*b_{k-j} and m_j*p_{k-j} into a 96-bit accumulator, so each
 // partial product costs a v_mad_u64_u32 plus one carry add and no moves.
-template <class P>
+//
+// With kReduce = false the final subtraction is skipped: for a*b < 4p^2
+// (say a < 4p and b < p, or both below 2p) and 4p < 2^256 the result is
+// below 2p, which lazily reduced transforms keep as their invariant.
+template <class P, bool kReduce = true>
 NKR_DEV Fp<P> fp_mul_fips(const Fp<P> &a, const Fp<P> &b)
 {
     uint32_t m[8];
@@ -258,7 +262,10 @@ NKR_DEV Fp<P> fp_mul_fips(const Fp<P> &a, const Fp<P> &b)
     Fp<P> r;
 #pragma unroll
     for (int j = 0; j < 8; ++j) r.l[j] = t[j];
-    return fp_reduce_once(r);
+    if constexpr (kReduce)
+        return fp_reduce_once(r);
+    else
+        return r;
 }
 
 // Two independent mac96 steps in one block: each v_mad_u64_u32 has the

@@ -52,6 +52,83 @@ __device__ __forceinline__ fr_t tw_pow(const PassArgs &a, uint32_t e)
     return fp_mul(a.tw_hi[hi], a.tw_lo[lo]);
 }
 
+// Lazy reduction: inside a lazily reduced pass every value is kept below 2r
+// (r < 2^254, so 4r < 2^256 and sums of two such values fit the eight words);
+// products skip their final subtraction (below 2r again when the other factor
+// is a canonical twiddle), and the pass stores canonical values.
+
+// x - 2r if x >= 2r, for x < 4r.
+__device__ __forceinline__ fr_t sub_twice_r_if(const fr_t &x)
+{
+    fr_t d;
+    uint64_t borrow = 0, c2 = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t two = 2ull * FrParams::P[i] + c2;
+        c2 = two >> 32;
+        const uint64_t t = (uint64_t)x.l[i] - (uint32_t)two - borrow;
+        d.l[i] = (uint32_t)t;
+        borrow = (t >> 63) & 1;
+    }
+    fr_t r;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) r.l[i] = borrow ? x.l[i] : d.l[i];
+    return r;
+}
+
+// x + y for x, y < 2r, reduced below 2r.
+__device__ __forceinline__ fr_t add_lazy(const fr_t &x, const fr_t &y)
+{
+    fr_t s;
+    uint64_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t t = (uint64_t)x.l[i] + y.l[i] + carry;
+        s.l[i] = (uint32_t)t;
+        carry = t >> 32;
+    }
+    return sub_twice_r_if(s);
+}
+
+// x - y + 2r for x, y < 2r: in (0, 4r), not reduced.
+__device__ __forceinline__ fr_t sub_lazy(const fr_t &x, const fr_t &y)
+{
+    fr_t s;
+    uint64_t carry = 0, c2 = 0, borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t two = 2ull * FrParams::P[i] + c2;
+        c2 = two >> 32;
+        const uint64_t t = (uint64_t)x.l[i] + (uint32_t)two + carry;
+        s.l[i] = (uint32_t)t;
+        carry = t >> 32;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t t = (uint64_t)s.l[i] - y.l[i] - borrow;
+        s.l[i] = (uint32_t)t;
+        borrow = (t >> 63) & 1;
+    }
+    return s;
+}
+
+// Decimation-in-frequency passes run lazily (their butterfly's difference
+// goes straight into the product); decimation-in-time passes stay canonical,
+// since their lazy butterfly needs a second conditional subtraction that
+// costs what the product saves (measured on MI355X).
+template <bool DIT>
+constexpr bool kLazyNtt = !DIT;
+
+// x * w * 2^-256 for x < 4r and a canonical w: below 2r.
+__device__ __forceinline__ fr_t mul_lazy(const fr_t &x, const fr_t &w)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return fp_mul_fips<FrParams, false>(x, w);
+#else
+    return fp_mul(x, w);
+#endif
+}
+
 // LDS tile of n field elements: two planes of 16-byte halves (element i's
 // low half in plane 0, high half in plane 1) so that every access is one
 // 16-byte LDS instruction per half, and within each aligned run of 64
@@ -88,12 +165,12 @@ __device__ __forceinline__ void lds_put(uint4 *tile, uint32_t n, uint32_t i, con
 // bits [np-t-R, np-t) that the round's butterflies pair.  The twiddle of the
 // pair (u, u + 2^(R-1-k)) at stage s = t + k is root^((j mod 2^(np-1-s)) * 2^s)
 // for its lower index j, as in the radix-2 DIF recursion.
-template <bool DIT, int R>
+template <bool DIT, int R, bool LOW0>
 __device__ __forceinline__ void ntt_round(uint4 *smem, uint32_t n, const PassArgs &a, uint32_t t, uint32_t cols,
                                           uint32_t root_shift)
 {
     constexpr uint32_t G = 1u << R;
-    const uint32_t low = a.np - t - R;
+    const uint32_t low = LOW0 ? 0u : a.np - t - R;
     const uint32_t groups = (1u << (a.np - R)) << a.log_cols;
     for (uint32_t gi = threadIdx.x; gi < groups; gi += blockDim.x) {
         const uint32_t c = gi & (cols - 1);
@@ -114,13 +191,30 @@ __device__ __forceinline__ void ntt_round(uint4 *smem, uint32_t n, const PassArg
                 const uint32_t e = (lo_part + ((u & (hs - 1u)) << low)) << s;
                 const fr_t w = a.roots[e << root_shift];
                 const fr_t X = x[u], Y = x[u + hs];
+                // twiddle 1 needs no product.  In the round whose pairs are
+                // adjacent (low = 0, so lo_part = 0) e depends on u alone and
+                // the test folds away; other rounds multiply every pair.
+                const bool one = LOW0 && (u & (hs - 1u)) == 0;
                 if (!DIT) {
-                    x[u] = fp_add(X, Y);
-                    x[u + hs] = fp_mul(fp_sub(X, Y), w);
+                    if (kLazyNtt<DIT>) {
+                        x[u] = add_lazy(X, Y);
+                        const fr_t D = sub_lazy(X, Y);
+                        x[u + hs] = one ? sub_twice_r_if(D) : mul_lazy(D, w);
+                    } else {
+                        x[u] = fp_add(X, Y);
+                        const fr_t D = fp_sub(X, Y);
+                        x[u + hs] = one ? D : fp_mul(D, w);
+                    }
                 } else {
-                    const fr_t wy = fp_mul(Y, w);
-                    x[u] = fp_add(X, wy);
-                    x[u + hs] = fp_sub(X, wy);
+                    if (kLazyNtt<DIT>) {
+                        const fr_t wy = one ? Y : mul_lazy(Y, w);
+                        x[u] = add_lazy(X, wy);
+                        x[u + hs] = sub_twice_r_if(sub_lazy(X, wy));
+                    } else {
+                        const fr_t wy = one ? Y : fp_mul(Y, w);
+                        x[u] = fp_add(X, wy);
+                        x[u + hs] = fp_sub(X, wy);
+                    }
                 }
             }
         }
@@ -129,8 +223,10 @@ __device__ __forceinline__ void ntt_round(uint4 *smem, uint32_t n, const PassArg
     }
 }
 
+// Radix-4 passes are held to 128 registers, four waves per SIMD: as many as
+// the LDS tiles allow (two 64 KiB tiles per CU), at the price of a few spills.
 template <bool DIT, int RMAX>
-__global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
+__global__ __launch_bounds__(kMaxThreads, RMAX == 2 ? 4 : 1) void k_ntt_pass(PassArgs a)
 {
     extern __shared__ uint4 smem[];
     if (a.batch) a.data = a.batch[blockIdx.y];
@@ -155,10 +251,10 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
         const uint32_t lo = (uint32_t)(g & (L - 1));
         const uint64_t pos = (hi << (a.log_n - a.S)) + ((uint64_t)j << a.log_L) + lo;
         fr_t v = a.data[pos];
-        if (a.pre) v = fp_mul(v, a.pre[pos]);
+        if (a.pre) v = kLazyNtt<DIT> ? mul_lazy(v, a.pre[pos]) : fp_mul(v, a.pre[pos]);
         if (DIT && a.twiddle) {
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
-            if (e) v = fp_mul(v, tw_pow(a, e));
+            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_pow(a, e)) : fp_mul(v, tw_pow(a, e));
         }
         lds_put(smem, total, j * cols + c, v);
     }
@@ -172,12 +268,23 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
     for (uint32_t done = 0; done < a.np;) {
         const uint32_t r = min((uint32_t)RMAX, a.np - done);
         const uint32_t t = DIT ? (a.np - done - r) : done;
-        if (r == 3)
-            ntt_round<DIT, 3>(smem, total, a, t, cols, root_shift);
-        else if (r == 2)
-            ntt_round<DIT, 2>(smem, total, a, t, cols, root_shift);
-        else
-            ntt_round<DIT, 1>(smem, total, a, t, cols, root_shift);
+        const bool low0 = (a.np - t - r == 0);
+        if (r == 3) {
+            if (low0)
+                ntt_round<DIT, 3, true>(smem, total, a, t, cols, root_shift);
+            else
+                ntt_round<DIT, 3, false>(smem, total, a, t, cols, root_shift);
+        } else if (r == 2) {
+            if (low0)
+                ntt_round<DIT, 2, true>(smem, total, a, t, cols, root_shift);
+            else
+                ntt_round<DIT, 2, false>(smem, total, a, t, cols, root_shift);
+        } else {
+            if (low0)
+                ntt_round<DIT, 1, true>(smem, total, a, t, cols, root_shift);
+            else
+                ntt_round<DIT, 1, false>(smem, total, a, t, cols, root_shift);
+        }
         __syncthreads();
         done += r;
     }
@@ -198,11 +305,11 @@ __global__ __launch_bounds__(kMaxThreads) void k_ntt_pass(PassArgs a)
         fr_t v = lds_get(smem, total, j * cols + c);
         if (!DIT && a.twiddle) {
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
-            if (e) v = fp_mul(v, tw_pow(a, e));
+            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_pow(a, e)) : fp_mul(v, tw_pow(a, e));
         }
-        if (a.post) v = fp_mul(v, a.post[pos]);
-        if (a.has_post_const) v = fp_mul(v, a.post_const);
-        a.data[pos] = v;
+        if (a.post) v = kLazyNtt<DIT> ? mul_lazy(v, a.post[pos]) : fp_mul(v, a.post[pos]);
+        if (a.has_post_const) v = kLazyNtt<DIT> ? mul_lazy(v, a.post_const) : fp_mul(v, a.post_const);
+        a.data[pos] = kLazyNtt<DIT> ? fp_reduce_once(v) : v;
     }
 }
 
