@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/nkr_g16.h"
+#include "host_pool.h"
 #include "msm.h"
 #include "ntt.h"
 #include "spmv.h"
@@ -92,7 +93,10 @@ private:
                 void *raw_out) const;
     // The two halves of finish: bs2 (and B) from the G2 planes, the rest from the G1 planes.
     void finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre, void *raw_out) const;
-    void finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out) const;
+    // helpers: two host threads to split the G1 tail over (null: all on
+    // the calling thread, as the batch path's per-proof workers do).
+    void finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out,
+                   HostHelpers *helpers) const;
     friend class BatchProver;
 
     // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)
@@ -102,6 +106,7 @@ private:
     std::vector<Affine<fq2_t>> tab_delta2_;
     g1_jac alpha1_, beta1_;
     g2_jac beta2_;
+    std::unique_ptr<HostHelpers> helpers_;  // the plain path's host tail
     StageTimer timer_;
     double stage_ms_[kStages] = {};
     uint64_t counters_[4] = {};

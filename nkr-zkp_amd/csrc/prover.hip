@@ -2,8 +2,8 @@
 
 #include <chrono>
 #include <cstring>
-#include <future>
 
+#include "glv.h"
 #include "hquery.h"
 #include "setup.h"
 
@@ -153,6 +153,7 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     planes2_.allocate(wplan_.c);
     NKR_HIP(hipHostMalloc(&pinned_witness_, (size_t)cols_ * sizeof(fr_t), hipHostMallocDefault));
     NKR_HIP(hipStreamSynchronize(stream_));
+    helpers_ = std::make_unique<HostHelpers>(2);
 }
 
 Groth16Prover::~Groth16Prover()
@@ -225,35 +226,6 @@ void Groth16Prover::prove_blind(const void *witness, bool on_device, const fr_t 
     run(z, raw_out, nullptr, nullptr, nullptr, nullptr, &job);
 }
 
-namespace {
-
-// a*P + b*Q with shared doublings (Straus, 4-bit windows).
-template <class F>
-Xyzz<F> straus2(const Xyzz<F> &P, const fr_t &a_mont, const Xyzz<F> &Q, const fr_t &b_mont)
-{
-    const fr_t a = fp_from_mont(a_mont), b = fp_from_mont(b_mont);
-    Xyzz<F> tp[16], tq[16];
-    tp[0] = tq[0] = Xyzz<F>::infinity();
-    tp[1] = P;
-    tq[1] = Q;
-    for (int i = 2; i < 16; ++i) {
-        tp[i] = xyzz_add(tp[i - 1], P);
-        tq[i] = xyzz_add(tq[i - 1], Q);
-    }
-    Xyzz<F> acc = Xyzz<F>::infinity();
-    for (int nib = 63; nib >= 0; --nib) {
-        if (!acc.is_inf())
-            for (int d = 0; d < 4; ++d) acc = xyzz_dbl(acc);
-        const uint32_t sh = (nib & 7) * 4;
-        const uint32_t va = (a.l[nib >> 3] >> sh) & 15u, vb = (b.l[nib >> 3] >> sh) & 15u;
-        if (va) acc = xyzz_add(acc, tp[va]);
-        if (vb) acc = xyzz_add(acc, tq[vb]);
-    }
-    return acc;
-}
-
-}  // namespace
-
 // A0 = alpha_1 + r delta_1, B0 = beta_2 + s delta_2 and
 // C0 = s alpha_1 + r beta_1 + (r s) delta_1, using
 // s*A + r*B_1 - (r s) delta_1 = s alpha_1 + s ar + r beta_1 + r bs1 + (r s) delta_1.
@@ -273,7 +245,7 @@ void Groth16Prover::finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const Bl
                            void *raw_out) const
 {
     finish_g2(p2, blind, pre, raw_out);
-    finish_g1(p1, blind, pre, raw_out);
+    finish_g1(p1, blind, pre, raw_out, nullptr);
 }
 
 void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
@@ -286,28 +258,40 @@ void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, cons
     std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 96, &B, 192);
 }
 
-void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre,
-                              void *raw_out) const
+void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out,
+                              HostHelpers *helpers) const
 {
     const uint32_t cw = wplan_.c;
     uint8_t *out = static_cast<uint8_t *>(raw_out);
-    // zk and H on a helper thread while this one does ar, bs1 and the r,s
-    // products that need them
-    g1_jac zk, qz;
-    auto other = std::async(std::launch::async, [&] {
+    // Three independent parts: zk and H; bs1 and r*bs1; ar, A and s*ar.
+    g1_jac zk, qz, ar, bs1;
+    g1_xyzz s_ar = g1_xyzz::infinity(), r_bs1 = g1_xyzz::infinity();
+    auto part_zq = [&] {
         zk = msm_finish(p1 + 2 * cw, cw);
         qz = msm_finish(p1 + 3 * cw, hplan_.c);
-    });
-    const g1_jac ar = msm_finish(p1 + 0 * cw, cw);
-    const g1_jac bs1 = msm_finish(p1 + 1 * cw, cw);
-    g1_xyzz rs_part;
+    };
+    auto part_b = [&] {
+        bs1 = msm_finish(p1 + 1 * cw, cw);
+        if (blind) r_bs1 = g1_mul_glv(jacobian_to_xyzz(bs1), blind->r);
+    };
+    if (helpers) {
+        helpers->run(0, part_zq);
+        helpers->run(1, part_b);
+    }
+    ar = msm_finish(p1 + 0 * cw, cw);
     if (blind) {
-        const g1_xyzz xar = jacobian_to_xyzz(ar), xbs1 = jacobian_to_xyzz(bs1);
+        const g1_xyzz xar = jacobian_to_xyzz(ar);
         const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(pre->A0, xar));
         std::memcpy(static_cast<uint8_t *>(blind->proof_out), &A, 96);
-        rs_part = straus2(xar, blind->s, xbs1, blind->r);
+        s_ar = g1_mul_glv(xar, blind->s);
     }
-    other.get();
+    if (helpers) {
+        helpers->wait(1);
+        helpers->wait(0);
+    } else {
+        part_b();
+        part_zq();
+    }
     std::memcpy(out + 0, &ar, 96);
     std::memcpy(out + 96, &bs1, 96);
     std::memcpy(out + 192, &zk, 96);
@@ -315,7 +299,7 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const
     if (!blind) return;
     g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
     C = xyzz_add(C, pre->C0);
-    C = xyzz_add(C, rs_part);
+    C = xyzz_add(C, xyzz_add(s_ar, r_bs1));
     const g1_jac Cj = xyzz_to_jacobian_affine(C);
     std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 288, &Cj, 96);
 }
@@ -326,6 +310,7 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     StageTimer *t = timer_.enabled() ? &timer_ : nullptr;
 
     const uint32_t cw = wplan_.c;
+    helpers_->arm();  // the host tail's helpers spin from here instead of sleeping
     NKR_HIP(hipEventRecord(ev_start_, s));
 
     // The quotient chain is the critical path: its launches are queued
@@ -394,7 +379,7 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
     planes1_host_.wait();
     host0 = std::chrono::steady_clock::now();
-    finish_g1(h1, blind, blind ? &pre : nullptr, raw_out);
+    finish_g1(h1, blind, blind ? &pre : nullptr, raw_out, helpers_.get());
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
 
     if (t) {
