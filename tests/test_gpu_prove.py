@@ -33,7 +33,7 @@ def test_prove_golden_matmul2():
     assert np.array_equal(pv.blind(raw, e.r, e.s), e.proof)
     # proving twice with the same context gives the same bytes
     assert np.array_equal(pv.prove(e.witness), raw)
-    # fused prove + blind (comb tables, Straus) gives the same proof
+    # fused prove + blind (comb tables, GLV scalar products) gives the same proof
     raw2, proof2 = pv.prove_blind(e.witness, e.r, e.s)
     assert np.array_equal(raw2, e.raw) and np.array_equal(proof2, e.proof)
     pv.close()
