@@ -2,6 +2,7 @@
 
 #include <chrono>
 #include <cstring>
+#include <exception>
 
 #include "glv.h"
 #include "hquery.h"
@@ -274,9 +275,23 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const
         bs1 = msm_finish(p1 + 1 * cw, cw);
         if (blind) r_bs1 = g1_mul_glv(jacobian_to_xyzz(bs1), blind->r);
     };
+    // the helpers' tasks refer to this frame: if anything below throws, the
+    // guard waits for them before the frame goes away
+    struct Join {
+        HostHelpers *h;
+        ~Join()
+        {
+            if (!h) return;
+            for (int i = 1; i >= 0; --i) try {
+                    h->wait(i);
+                } catch (...) {
+                }
+        }
+    } join{nullptr};
     if (helpers) {
         helpers->run(0, part_zq);
         helpers->run(1, part_b);
+        join.h = helpers;
     }
     ar = msm_finish(p1 + 0 * cw, cw);
     if (blind) {
@@ -286,8 +301,14 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const
         s_ar = g1_mul_glv(xar, blind->s);
     }
     if (helpers) {
-        helpers->wait(1);
-        helpers->wait(0);
+        join.h = nullptr;
+        std::exception_ptr err;
+        for (int i = 1; i >= 0; --i) try {
+                helpers->wait(i);
+            } catch (...) {
+                if (!err) err = std::current_exception();
+            }
+        if (err) std::rethrow_exception(err);
     } else {
         part_b();
         part_zq();
