@@ -127,9 +127,18 @@ public:
         NKR_HIP(hipEventRecord(event_, stream));
     }
     // Blocks until those transfers are done (a download is then readable).
+    // Polls instead of sleeping in the runtime: the proof's host tail waits
+    // here for a few kilobytes, and a sleeping thread's wake-up would add
+    // tens of microseconds to every proof.
     void wait() const
     {
-        if (event_) NKR_HIP(hipEventSynchronize(event_));
+        if (!event_) return;
+        for (;;) {
+            const hipError_t e = hipEventQuery(event_);
+            if (e == hipSuccess) return;
+            if (e != hipErrorNotReady) NKR_HIP(e);
+            for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
+        }
     }
     T *get() const { return ptr_; }
 
