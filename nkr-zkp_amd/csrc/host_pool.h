@@ -17,6 +17,8 @@
 #include <thread>
 #include <vector>
 
+#include "runtime.h"
+
 namespace nkr {
 
 class HostHelpers {
@@ -111,7 +113,7 @@ private:
                     }
                     while ((st = s.state.load(std::memory_order_acquire)) != kReady && st != kStop &&
                            std::chrono::steady_clock::now() < until)
-                        for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
+                        cpu_relax();
                     s.armed.store(false, std::memory_order_release);
                 }
                 if (st != kReady && st != kStop) {

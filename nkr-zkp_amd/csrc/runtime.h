@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -93,6 +94,16 @@ private:
 
 // Pinned host staging buffer for asynchronous uploads.  `reuse` waits for the
 // upload recorded by `uploaded` before the host writes the buffer again.
+// A short pause for host spin loops.
+inline void cpu_relax()
+{
+#if defined(__x86_64__) || defined(__i386__)
+    for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
+#else
+    std::this_thread::yield();
+#endif
+}
+
 template <class T>
 class PinnedStaging {
 public:
@@ -137,7 +148,7 @@ public:
             const hipError_t e = hipEventQuery(event_);
             if (e == hipSuccess) return;
             if (e != hipErrorNotReady) NKR_HIP(e);
-            for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
+            cpu_relax();
         }
     }
     T *get() const { return ptr_; }
