@@ -138,15 +138,34 @@ public:
     // Sort workgroups for n scalars (each histograms a contiguous slice).
     static uint32_t sort_blocks(uint32_t n);
 
+    // A second split of the same sorted entries into pieces of at least
+    // `piece_lo` entries (chosen on the device as in build, within
+    // [piece_lo, max(piece_lo, plan piece_hi)]), computed by the next builds:
+    // one scalar vector feeds MSMs in different groups, and a G1 MSM
+    // completes its buckets faster from fewer, longer pieces while the G2
+    // MSM, at one wave per SIMD, needs the short ones.  0 turns it off.
+    // Jobs select it with MsmJob::alt.
+    void set_alt_pieces(uint32_t piece_lo) { alt_piece_lo_ = piece_lo; }
+    uint64_t alt_max_pieces() const
+    {
+        MsmPlan p = plan_;
+        p.piece_lo = alt_piece_lo_;
+        p.piece_hi = std::max(alt_piece_lo_, p.piece_hi);
+        return p.max_pieces(target_log_);
+    }
+    const uint32_t *alt_totals_host() const { return totals_host() + 4; }
+
     MsmPlan plan_;
     uint32_t blocks_ = 0;
     uint32_t target_log_ = kMsmTargetPiecesLog;  // as of the last prepare
     DeviceBuffer<uint32_t> sorted_;        // entries: sign << 31 | (w * n + i)
     DeviceBuffer<uint32_t> count_;         // per bucket (B)
-    DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the totals above
+    DeviceBuffer<uint32_t> bucket_start_;  // B + 1, then the totals above, then the second split's
     DeviceBuffer<uint32_t> piece_start_;   // B + 1
     DeviceBuffer<uint32_t> block_count_;   // blocks x B: per-workgroup counts, then offsets
     PinnedStaging<uint32_t> own_totals_;   // host copy of the counters (build)
+    uint32_t alt_piece_lo_ = 0;
+    DeviceBuffer<uint32_t> alt_piece_start_;  // B + 1 (its counters follow the first split's)
     const uint32_t *host_totals_ = nullptr;
     const PinnedStaging<uint32_t> *host_stage_ = nullptr;
 };
@@ -177,7 +196,8 @@ template <class F>
 struct MsmJob {
     const MsmBases<F> *bases;
     const MsmSchedule *sched;
-    Xyzz<F> *planes;  // device output: c points U_0 .. U_{c-1}
+    Xyzz<F> *planes;   // device output: c points U_0 .. U_{c-1}
+    bool alt = false;  // use the schedule's second piece split (set_alt_pieces)
 };
 
 // Runs a batch of MSMs of one group (all with the same window width).

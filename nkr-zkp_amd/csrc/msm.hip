@@ -259,12 +259,29 @@ __device__ void schedule_scan(const uint32_t *__restrict__ count, uint32_t B, ui
     }
 }
 
+// One schedule's bucket starts and piece split; with alt_lo > 0 also the
+// second split (pieces of at least alt_lo entries) into alt_piece_start and
+// the counters bucket_start[B + 4 .. B + 8).
 __global__ __launch_bounds__(1024) void k_schedule_scan(const uint32_t *__restrict__ count, uint32_t B,
                                                         uint32_t piece_lo, uint32_t piece_hi, uint32_t target_log,
                                                         uint32_t *__restrict__ bucket_start,
-                                                        uint32_t *__restrict__ piece_start)
+                                                        uint32_t *__restrict__ piece_start, uint32_t alt_lo,
+                                                        uint32_t *__restrict__ alt_piece_start)
 {
     schedule_scan(count, B, piece_lo, piece_hi, target_log, bucket_start, piece_start);
+    if (alt_lo == 0) return;
+    const uint32_t entries = bucket_start[B];
+    const uint32_t q = entries >> target_log;
+    const uint32_t hi = max(alt_lo, piece_hi);
+    uint32_t piece = alt_lo;
+    while (piece < hi && 2 * piece <= q) piece <<= 1;
+    uint32_t *ctr = bucket_start + B + 4;
+    const uint32_t pieces = block_scan(count, B, 2, piece, 0, alt_piece_start, ctr + 2);
+    if (threadIdx.x == 0) {
+        ctr[0] = entries;
+        ctr[1] = pieces;
+        ctr[3] = piece;
+    }
 }
 
 // Last index k in [0, len) with starts[k] <= p (starts[0] <= p < starts[len]).
@@ -800,7 +817,7 @@ uint32_t MsmSchedule::prepare(const MsmPlan &plan)
     blocks_ = sort_blocks(n);
     sorted_.ensure((size_t)n * plan.W);
     count_.ensure(B);
-    bucket_start_.ensure(B + 4);  // + pieces, largest piece count of a bucket, piece length
+    bucket_start_.ensure(B + 8);  // + the four counters of each piece split
     piece_start_.ensure(B + 1);
     block_count_.ensure((size_t)blocks_ * B);
     return div_up(n, blocks_);
@@ -817,11 +834,13 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
     NKR_LAUNCH_CHECK();
     k_colscan<<<div_up(B, 64), 64 * kScanGroups, 0, stream>>>(block_count_.get(), blocks_, B, count_.get());
     NKR_LAUNCH_CHECK();
+    if (alt_piece_lo_) alt_piece_start_.ensure(B + 1);
     k_schedule_scan<<<1, 1024, 0, stream>>>(count_.get(), B, plan.piece_lo, plan.piece_hi, target_log_,
-                                            bucket_start_.get(), piece_start_.get());
+                                            bucket_start_.get(), piece_start_.get(), alt_piece_lo_,
+                                            alt_piece_lo_ ? alt_piece_start_.get() : nullptr);
     NKR_LAUNCH_CHECK();
-    uint32_t *th = own_totals_.reuse(4);
-    NKR_HIP(hipMemcpyAsync(th, bucket_start_.get() + B, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    uint32_t *th = own_totals_.reuse(8);
+    NKR_HIP(hipMemcpyAsync(th, bucket_start_.get() + B, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     own_totals_.uploaded(stream);
     host_totals_ = th;
     host_stage_ = &own_totals_;
@@ -847,6 +866,7 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     for (uint32_t j = 0; j < J; ++j) {
         MsmSchedule &s = *items[j].sched;
         const MsmPlan &p = items[j].plan;
+        NKR_REQUIRE(s.alt_piece_lo_ == 0, "batched schedules have one piece split");
         const uint32_t per_block = s.prepare(p);
         jobs[j] = SortJob{items[j].scalars_mont, p.n, p.c, p.W, p.B, s.blocks_, per_block, p.piece_lo, p.piece_hi,
                           s.target_log_, s.count_.get(), s.block_count_.get(), s.bucket_start_.get(), s.piece_start_.get(),
@@ -884,6 +904,23 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     NKR_LAUNCH_CHECK();
 }
 
+// The piece split a job reads: its schedule's own, or the second one.
+template <class F>
+static const uint32_t *job_counters(const MsmJob<F> &jb)
+{
+    return jb.sched->bucket_start_.get() + jb.sched->plan().B + (jb.alt ? 4 : 0);
+}
+template <class F>
+static const uint32_t *job_piece_start(const MsmJob<F> &jb)
+{
+    return jb.alt ? jb.sched->alt_piece_start_.get() : jb.sched->piece_start_.get();
+}
+template <class F>
+static uint64_t job_max_pieces(const MsmJob<F> &jb)
+{
+    return jb.alt ? jb.sched->alt_max_pieces() : jb.sched->max_pieces();
+}
+
 template <class F>
 void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
 {
@@ -892,9 +929,9 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         key.push_back(jb.bases->data());
         key.push_back(jb.sched->sorted_.get());
         key.push_back(jb.sched->bucket_start_.get());
-        key.push_back(jb.sched->piece_start_.get());
+        key.push_back(job_piece_start(jb));
         key.push_back(jb.planes);
-        key.push_back(reinterpret_cast<const void *>((uintptr_t)jb.sched->max_pieces()));
+        key.push_back(reinterpret_cast<const void *>((uintptr_t)job_max_pieces(jb)));
     }
     if (key == key_) return;
     const uint32_t J = (uint32_t)jobs.size();
@@ -909,7 +946,7 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         const MsmPlan &p = jobs[j].sched->plan();
         NKR_REQUIRE(jobs[j].bases->plan().n == p.n && jobs[j].bases->plan().c == p.c,
                     "MSM bases and schedule disagree");
-        const uint64_t pieces = jobs[j].sched->max_pieces();
+        const uint64_t pieces = job_max_pieces(jobs[j]);
         thread_base_[j + 1] = thread_base_[j] + pieces;
         uint64_t cap = pieces, cnt = pieces, base = pieces;
         uint32_t levels = 0;
@@ -946,9 +983,9 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         d.bases = jb.bases->data();
         d.sorted = jb.sched->sorted_.get();
         d.bucket_start = jb.sched->bucket_start_.get();
-        d.piece_start = jb.sched->piece_start_.get();
-        d.piece_max = jb.sched->bucket_start_.get() + jb.sched->plan().B + 2;
-        d.piece_len = jb.sched->bucket_start_.get() + jb.sched->plan().B + 3;
+        d.piece_start = job_piece_start(jb);
+        d.piece_max = job_counters(jb) + 2;
+        d.piece_len = job_counters(jb) + 3;
         d.pool = pool_.get() + pool_off[j];
         d.reg_start = reg_start_.get() + b_off[j];
         d.reg_count = reg_count_.get() + b_off[j];
@@ -1008,7 +1045,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     // so this wait overlaps the GPU's work instead of stalling it.
     uint32_t folds = 0;
     for (const auto &jb : jobs) {
-        uint64_t cnt = jb.sched->totals_host()[2];
+        uint64_t cnt = (jb.alt ? jb.sched->alt_totals_host() : jb.sched->totals_host())[2];
         uint32_t levels = 0;
         while (cnt > kMsmCompleteMax) {
             cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;

@@ -155,6 +155,11 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     NKR_HIP(hipHostMalloc(&pinned_witness_, (size_t)cols_ * sizeof(fr_t), hipHostMallocDefault));
     NKR_HIP(hipStreamSynchronize(stream_));
     helpers_ = std::make_unique<HostHelpers>(2);
+    // The witness G1 MSMs read the schedule in pieces of at least 32 entries
+    // (a few partial sums per bucket to complete); bs2 keeps the short
+    // device-chosen pieces its one-wave-per-SIMD accumulation needs.
+    // Measured on the 2^20 prove: 4.13 -> 3.98 ms.
+    wsched_.set_alt_pieces(32);
 }
 
 Groth16Prover::~Groth16Prover()
@@ -379,9 +384,9 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     // latency-bound reduction levels are paid once for all four.
     hsched_.build(pa_.get(), hplan_, s, t);
     NKR_HIP(hipStreamWaitEvent(s, ev_wsched_, 0));
-    batch_g1_.run({MsmJob<fq_t>{&a_, &wsched_, planes1_.get() + 0 * cw},
-                   MsmJob<fq_t>{&b1_, &wsched_, planes1_.get() + 1 * cw},
-                   MsmJob<fq_t>{&l_, &wsched_, planes1_.get() + 2 * cw},
+    batch_g1_.run({MsmJob<fq_t>{&a_, &wsched_, planes1_.get() + 0 * cw, true},
+                   MsmJob<fq_t>{&b1_, &wsched_, planes1_.get() + 1 * cw, true},
+                   MsmJob<fq_t>{&l_, &wsched_, planes1_.get() + 2 * cw, true},
                    MsmJob<fq_t>{&h_, &hsched_, planes1_.get() + 3 * cw}},
                   s, t);
     const size_t n1 = 3 * cw + hplan_.c;
