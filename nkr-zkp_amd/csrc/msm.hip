@@ -214,16 +214,27 @@ __device__ uint32_t block_scan(const uint32_t *__restrict__ src, uint32_t len, u
         sum += v;
         mx = max(mx, v);
     }
-    part[threadIdx.x] = sum;
     if (max_out && mx) atomicMax(&pmax, mx);
-    __syncthreads();
-    for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
-        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += t;
-        __syncthreads();
+    // the runs' sums scanned across the workgroup: within each wave by
+    // shuffles, then the waves' totals by the first wave (two barriers)
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= (uint32_t)off) incl += y;
     }
-    uint32_t run = part[threadIdx.x] - sum;
+    if (lane == 63) part[wave] = incl;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < waves ? part[lane] : 0u;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)w, off);
+            if (lane >= (uint32_t)off) w += y;
+        }
+        if (lane < waves) part[lane] = w;
+    }
+    __syncthreads();
+    uint32_t run = (wave ? part[wave - 1] : 0u) + incl - sum;
     for (uint32_t k = 0; k < end - beg; ++k) {
         const uint32_t v = run_vals[k];
         run_vals[k] = run;
@@ -231,7 +242,7 @@ __device__ uint32_t block_scan(const uint32_t *__restrict__ src, uint32_t len, u
     }
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < len; k += blockDim.x) out[k] = vals[at(k)];
-    const uint32_t total = part[blockDim.x - 1];
+    const uint32_t total = part[waves - 1];
     if (threadIdx.x == 0) {
         out[len] = total;
         if (max_out) *max_out = pmax;
