@@ -87,6 +87,26 @@ typedef struct nkr_g16_ctx nkr_g16_ctx;
 /* Uploads the circuit, normalises and window-expands the proving key in HBM
  * and builds the NTT tables (the reference's prepare).  Host pointers. */
 int nkr_g16_ctx_create(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int device, nkr_g16_ctx **out);
+
+/* Per-context tuning (0 in a field keeps the default).  The proof does not
+ * depend on any of them, only the time does; nkr-zkp_amd/autotune.py picks
+ * them per (domain size, witness density) by measurement.  No counterpart in
+ * the reference (its window and stage choices are fixed,
+ * operators/hip/include/msm/msm.hip:25-60).
+ *   witness_window_bits  MSM window of the witness MSMs, 2..15
+ *   h_window_bits        MSM window of the H MSM, 2..16
+ *   ntt_lds_log          10 or 11: 2^lds_log elements per transform tile
+ *   ntt_radix_log        2 or 3: register radix of the transform rounds
+ *   ntt_split_policy     1 + policy: 1 balanced passes, 2 full last pass
+ *   msm_target_pieces_log  2^k accumulation pieces per MSM launch, 12..24
+ *   witness_g1_piece_min   shortest piece of the witness G1 MSMs, 1..64 */
+typedef struct {
+    uint32_t witness_window_bits, h_window_bits;
+    uint32_t ntt_lds_log, ntt_radix_log, ntt_split_policy;
+    uint32_t msm_target_pieces_log, witness_g1_piece_min;
+} nkr_g16_tuning;
+int nkr_g16_ctx_create_tuned(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int device,
+                             const nkr_g16_tuning *tuning, nkr_g16_ctx **out);
 int nkr_g16_ctx_destroy(nkr_g16_ctx *ctx);
 /* Device bytes held by the context. */
 int nkr_g16_ctx_device_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes);

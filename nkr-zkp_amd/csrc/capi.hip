@@ -145,6 +145,19 @@ int nkr_g16_ctx_create(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int
     });
 }
 
+int nkr_g16_ctx_create_tuned(const nkr_g16_circuit *circuit, const nkr_g16_pk *pk, int device,
+                             const nkr_g16_tuning *tuning, nkr_g16_ctx **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(circuit && pk && out && tuning, "null argument");
+        *out = nullptr;
+        set_device(device);
+        auto ctx = std::make_unique<nkr_g16_ctx>();
+        ctx->prover = std::make_unique<Groth16Prover>(*circuit, *pk, device, tuning);
+        *out = ctx.release();
+    });
+}
+
 int nkr_g16_ctx_destroy(nkr_g16_ctx *ctx)
 {
     return guarded([&] { delete ctx; });

@@ -146,6 +146,9 @@ public:
     // MSM, at one wave per SIMD, needs the short ones.  0 turns it off.
     // Jobs select it with MsmJob::alt.
     void set_alt_pieces(uint32_t piece_lo) { alt_piece_lo_ = piece_lo; }
+    // 2^target_log level-0 pieces per build instead of the process-wide
+    // setting (0 restores it).
+    void set_target_log(uint32_t target_log) { target_override_ = target_log; }
     uint64_t alt_max_pieces() const
     {
         MsmPlan p = plan_;
@@ -165,6 +168,7 @@ public:
     DeviceBuffer<uint32_t> block_count_;   // blocks x B: per-workgroup counts, then offsets
     PinnedStaging<uint32_t> own_totals_;   // host copy of the counters (build)
     uint32_t alt_piece_lo_ = 0;
+    uint32_t target_override_ = 0;
     DeviceBuffer<uint32_t> alt_piece_start_;  // B + 1 (its counters follow the first split's)
     const uint32_t *host_totals_ = nullptr;
     const PinnedStaging<uint32_t> *host_stage_ = nullptr;

@@ -823,7 +823,7 @@ uint32_t MsmSchedule::prepare(const MsmPlan &plan)
     NKR_REQUIRE(plan.B <= kMsmMaxBuckets, "MSM window too wide for the bucket scans");
     NKR_REQUIRE(plan.piece_lo >= 1 && plan.piece_lo <= plan.piece_hi, "MSM piece bounds");
     plan_ = plan;
-    target_log_ = msm_target_pieces_log();
+    target_log_ = target_override_ ? target_override_ : msm_target_pieces_log();
     const uint32_t n = plan.n, B = plan.B;
     blocks_ = sort_blocks(n);
     sorted_.ensure((size_t)n * plan.W);

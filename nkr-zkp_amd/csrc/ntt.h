@@ -21,9 +21,17 @@
 
 namespace nkr {
 
+// Workgroup shape of the transform passes: 2^lds_log elements per LDS tile,
+// 2^radix_log per register round, pass split policy (0 balanced, 1 a full
+// unit-stride last pass).  Zero fields take the process-wide setting.
+struct NttShape {
+    unsigned lds_log = 0, radix_log = 0, split_policy = 0;
+    bool split_set = false;
+};
+
 class NttDomain {
 public:
-    NttDomain(unsigned log_n, hipStream_t stream);
+    NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape = nullptr);
 
     unsigned log_n() const { return log_n_; }
     size_t size() const { return size_t(1) << log_n_; }

@@ -410,7 +410,7 @@ fr_t fr_root_of_unity(unsigned log_n)
     return w;
 }
 
-NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
+NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) : log_n_(log_n)
 {
     NKR_REQUIRE(log_n >= 1 && log_n <= kTwoAdicity, "NTT size must be 2^1 .. 2^28");
     const size_t n = size_t(1) << log_n;
@@ -422,12 +422,16 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream) : log_n_(log_n)
     zinv_ = fp_inv(fp_sub(fr_pow_u64(g, n), fr_t::one()));
 
     // pass plan: sub-transforms of at most lds_log bits
-    lds_log_ = g_lds_log;
-    radix_log_ = g_radix_log;
+    lds_log_ = shape && shape->lds_log ? shape->lds_log : g_lds_log;
+    radix_log_ = shape && shape->radix_log ? shape->radix_log : g_radix_log;
+    const unsigned split_policy = shape && shape->split_set ? shape->split_policy : g_split_policy;
+    NKR_REQUIRE(lds_log_ == 10 || lds_log_ == 11, "LDS elements per workgroup must be 2^10 or 2^11");
+    NKR_REQUIRE(radix_log_ == 2 || radix_log_ == 3, "register radix must be 2^2 or 2^3");
+    NKR_REQUIRE(split_policy <= 1, "pass split policy must be 0 (balanced) or 1 (contiguous last)");
     std::vector<unsigned> split;
     if (log_n <= lds_log_) {
         split = {log_n};
-    } else if (g_split_policy == 1) {
+    } else if (split_policy == 1) {
         // The last pass (unit stride) takes a full LDS tile; the strided passes
         // before it take at most lds_log - 2 bits, so that each workgroup
         // gathers >= 4 adjacent columns (>= 128 contiguous bytes per row).

@@ -27,7 +27,9 @@ void check_circuit(const nkr_g16_circuit &c);
 
 class Groth16Prover {
 public:
-    Groth16Prover(const nkr_g16_circuit &circuit, const nkr_g16_pk &pk, int device);
+    // tuning: optional per-context choices (nkr_g16_tuning; null = defaults)
+    Groth16Prover(const nkr_g16_circuit &circuit, const nkr_g16_pk &pk, int device,
+                  const nkr_g16_tuning *tuning = nullptr);
     ~Groth16Prover();
     Groth16Prover(const Groth16Prover &) = delete;
     Groth16Prover &operator=(const Groth16Prover &) = delete;

@@ -60,8 +60,20 @@ void check_circuit(const nkr_g16_circuit &c)
     for (int m = 0; m < 3; ++m) NKR_REQUIRE(nnz[m] < (1ull << 32), "matrix has too many non-zeros");
 }
 
-Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int device) : device_(device)
+Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int device,
+                             const nkr_g16_tuning *tuning)
+    : device_(device)
 {
+    nkr_g16_tuning tn{};
+    if (tuning) tn = *tuning;
+    NKR_REQUIRE(tn.witness_window_bits == 0 || (tn.witness_window_bits >= 2 && tn.witness_window_bits <= 15),
+                "tuning: witness window must be 2..15 bits");
+    NKR_REQUIRE(tn.h_window_bits == 0 || (tn.h_window_bits >= 2 && tn.h_window_bits <= 16),
+                "tuning: H window must be 2..16 bits");
+    NKR_REQUIRE(tn.ntt_split_policy <= 2, "tuning: NTT split policy must be 0 (default), 1 or 2");
+    NKR_REQUIRE(tn.msm_target_pieces_log == 0 || (tn.msm_target_pieces_log >= 12 && tn.msm_target_pieces_log <= 24),
+                "tuning: MSM target pieces must be 2^12 .. 2^24");
+    NKR_REQUIRE(tn.witness_g1_piece_min <= kMsmPiece, "tuning: witness G1 piece minimum must be 0..64");
     // validate_circuit / validate_proving_key (protocols/groth16/common/src/protocol.cpp:9-67)
     check_circuit(c);
     NKR_REQUIRE(pk.a_query && pk.b_g1_query && pk.b_g2_query && pk.l_query && pk.h_query,
@@ -90,12 +102,19 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
         views_[m].long_rows = long_rows_[m].get();
     }
 
-    domain_ = std::make_unique<NttDomain>(log_n_, stream_);
+    NttShape shape;
+    shape.lds_log = tn.ntt_lds_log;
+    shape.radix_log = tn.ntt_radix_log;
+    shape.split_set = tn.ntt_split_policy != 0;
+    shape.split_policy = tn.ntt_split_policy ? tn.ntt_split_policy - 1 : 0;
+    domain_ = std::make_unique<NttDomain>(log_n_, stream_, &shape);
     // The witness schedule is sorted on the side stream beside the transforms:
     // at most 15-bit windows, whose 64 KiB histograms share a CU with them.
-    wplan_ = MsmPlan::choose(cols_);
+    wplan_ = MsmPlan::choose(cols_, tn.witness_window_bits);
     if (wplan_.c > 15) wplan_ = MsmPlan::choose(cols_, 15);
-    hplan_ = MsmPlan::choose(rows_);
+    hplan_ = MsmPlan::choose(rows_, tn.h_window_bits);
+    wsched_.set_target_log(tn.msm_target_pieces_log);
+    hsched_.set_target_log(tn.msm_target_pieces_log);
 
     {
         DeviceBuffer<g1_jac> tmp1(std::max(cols_, rows_));
@@ -159,7 +178,7 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     // (a few partial sums per bucket to complete); bs2 keeps the short
     // device-chosen pieces its one-wave-per-SIMD accumulation needs.
     // Measured on the 2^20 prove: 4.13 -> 3.98 ms.
-    wsched_.set_alt_pieces(32);
+    wsched_.set_alt_pieces(tn.witness_g1_piece_min ? tn.witness_g1_piece_min : 32);
 }
 
 Groth16Prover::~Groth16Prover()

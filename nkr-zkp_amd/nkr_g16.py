@@ -33,6 +33,20 @@ class Circuit(ctypes.Structure):
                 ("c_nnz", ctypes.c_uint64)]
 
 
+class Tuning(ctypes.Structure):
+    """nkr_g16_tuning: per-context choices, 0 = default (see include/nkr_g16.h)."""
+    FIELDS = ("witness_window_bits", "h_window_bits", "ntt_lds_log", "ntt_radix_log", "ntt_split_policy",
+              "msm_target_pieces_log", "witness_g1_piece_min")
+    _fields_ = [(n, ctypes.c_uint32) for n in FIELDS]
+
+    @classmethod
+    def of(cls, d) -> "Tuning":
+        unknown = set(d) - set(cls.FIELDS)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        return cls(**{k: int(v) for k, v in d.items()})
+
+
 class Pk(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
@@ -69,6 +83,8 @@ def lib() -> ctypes.CDLL:
         sigs = {
             "nkr_device_count": [ctypes.POINTER(ctypes.c_int)],
             "nkr_g16_ctx_create": [ctypes.POINTER(Circuit), ctypes.POINTER(Pk), i32, ctypes.POINTER(vp)],
+            "nkr_g16_ctx_create_tuned": [ctypes.POINTER(Circuit), ctypes.POINTER(Pk), i32, ctypes.POINTER(Tuning),
+                                         ctypes.POINTER(vp)],
             "nkr_g16_ctx_destroy": [vp],
             "nkr_g16_ctx_device_bytes": [vp, ctypes.POINTER(u64)],
             "nkr_g16_prove": [vp, vp, vp],
@@ -406,13 +422,19 @@ class Server:
 class Prover:
     """Groth16 backend context: prepare in the constructor, prove per witness."""
 
-    def __init__(self, circ, pk, device: int = 0):
+    def __init__(self, circ, pk, device: int = 0, tuning=None):
+        """tuning: optional dict of nkr_g16_tuning fields (autotune.py makes them)."""
         self._keep = []
         self.rows, self.cols = circ.rows, circ.cols
         c = _circuit_struct(circ, self._keep)
         self.pk_struct = self._pk(pk)
         h = ctypes.c_void_p()
-        _check(lib().nkr_g16_ctx_create(ctypes.byref(c), ctypes.byref(self.pk_struct), device, ctypes.byref(h)))
+        if tuning:
+            t = Tuning.of(tuning)
+            _check(lib().nkr_g16_ctx_create_tuned(ctypes.byref(c), ctypes.byref(self.pk_struct), device,
+                                                  ctypes.byref(t), ctypes.byref(h)))
+        else:
+            _check(lib().nkr_g16_ctx_create(ctypes.byref(c), ctypes.byref(self.pk_struct), device, ctypes.byref(h)))
         self.h = h
 
     def _pk(self, pk) -> Pk:
