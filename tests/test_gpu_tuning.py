@@ -19,7 +19,7 @@ TUNINGS = [
     {"msm_target_pieces_log": 12, "witness_g1_piece_min": 64},
     {"witness_window_bits": 13, "h_window_bits": 14, "ntt_split_policy": 2, "msm_target_pieces_log": 20,
      "witness_g1_piece_min": 1},
-]
+] + [t for t in A.TuningTable.shipped().entries.values() if t]  # every tuning the package ships
 
 
 def test_tuned_contexts_prove_the_same():
@@ -58,7 +58,8 @@ def test_autotune_search_keeps_the_proof():
     rs = O.fr_random(0x7E8, 2)
     knobs = {"h_window_bits": [0, 6], "witness_g1_piece_min": [0, 8]}
     best, report = A.autotune(circ, pk, w, rs[0], rs[1], proves=3, knobs=knobs)
-    assert len(report) == 3 and all(ms > 0 for _, ms in report)
+    assert len(report) >= 3 and all(ms > 0 for _, _, ms in report)
+    assert report[0][0] == "default" and not any(report[0][1].values())
     assert set(best) <= {"h_window_bits", "witness_g1_piece_min"}
     pv = N.Prover(circ, pk, tuning=best)
     raw = pv.prove(w)
