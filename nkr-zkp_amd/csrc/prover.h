@@ -57,6 +57,7 @@ public:
     const MsmPlan &h_plan() const { return hplan_; }
 
 private:
+    void release() noexcept;  // streams, events and pinned memory (destructor, failed constructor)
     int device_;
     hipStream_t stream_ = nullptr;
     uint32_t rows_, cols_, log_n_;

@@ -83,6 +83,17 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     cols_ = (uint32_t)c.cols;
     log_n_ = ceil_log2(rows_);
     NKR_HIP(hipSetDevice(device_));
+    // A constructor that throws never runs the destructor: hand the streams,
+    // events and pinned memory back if a later step (an upload, a prepare
+    // that runs out of device memory) throws.
+    struct ReleaseOnThrow {
+        Groth16Prover *self;
+        bool armed = true;
+        ~ReleaseOnThrow()
+        {
+            if (armed) self->release();
+        }
+    } release_on_throw{this};
     NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
     NKR_HIP(hipEventCreateWithFlags(&ev_start_, hipEventDisableTiming));
@@ -179,21 +190,31 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     // device-chosen pieces its one-wave-per-SIMD accumulation needs.
     // Measured on the 2^20 prove: 4.13 -> 3.98 ms.
     wsched_.set_alt_pieces(tn.witness_g1_piece_min ? tn.witness_g1_piece_min : 32);
+    release_on_throw.armed = false;
 }
 
 Groth16Prover::~Groth16Prover()
 {
+    release();
+}
+
+void Groth16Prover::release() noexcept
+{
     if (stream2_) {
         (void)hipStreamSynchronize(stream2_);
         (void)hipStreamDestroy(stream2_);
+        stream2_ = nullptr;
     }
     if (stream_) {
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
     }
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (ev_wsched_) (void)hipEventDestroy(ev_wsched_);
+    ev_start_ = ev_wsched_ = nullptr;
     if (pinned_witness_) (void)hipHostFree(pinned_witness_);
+    pinned_witness_ = nullptr;
 }
 
 uint64_t Groth16Prover::device_bytes() const

@@ -212,3 +212,93 @@ def test_msm_closed_form(log_n):
         want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
         assert np.array_equal(m.run(s), want), variant
     m.close()
+
+
+# ------------------------------------------- operator sizes: closed forms
+_RINV = pow(O.MONT, -1, O.R)
+
+
+def block_sums(s, block):
+    """Canonical sum of each run of `block` consecutive Montgomery-form
+    scalars.  Montgomery form is linear (s_i R mod r), so a run's sum is R^-1
+    times the sum of its words, mod r; the words are summed exactly in numpy
+    as 32-bit halves (under 2^32 terms per run, every column stays below 2^64)."""
+    a = np.ascontiguousarray(s, dtype=np.uint64).reshape(-1, block, 4)
+    lo = (a & np.uint64(0xFFFFFFFF)).sum(axis=1, dtype=np.uint64).tolist()
+    hi = (a >> np.uint64(32)).sum(axis=1, dtype=np.uint64).tolist()
+    out = []
+    for lr, hr in zip(lo, hi):
+        t = sum((lv << (64 * w)) + (hv << (64 * w + 32)) for w, (lv, hv) in enumerate(zip(lr, hr)))
+        out.append(t * _RINV % O.R)
+    return out
+
+
+def check_block_sums(s, t, block, runs=4):
+    f = O.fr_ints(s[: runs * block])
+    assert t[:runs] == [sum(f[b * block:(b + 1) * block]) % O.R for b in range(runs)]
+
+
+def witness_like(s, seed):
+    """Half of the scalars set to 0 or 1 (witness-like mix), the rest kept."""
+    rng = np.random.default_rng(seed)
+    pick = rng.random(s.shape[0]) < 0.5
+    bit = rng.random(s.shape[0]) < 0.5
+    s[pick & bit] = O.int_to_words(O.to_mont_int(1, O.R))
+    s[pick & ~bit] = 0
+    return s
+
+
+@pytest.mark.parametrize("log_n", [20, 24])
+def test_msm_g1_operator_sizes_closed_form(log_n):
+    """G1 MSM at 2^20 and 2^24 terms (c = 16, 16 windows: 2^28 bucket entries
+    and 16 GiB of window-shifted bases at 2^24), uniform and witness-like
+    scalars.  Bases k_j G repeat over runs of 64 terms, so
+    sum_i s_i B_i = (sum_j k_j T_j) G with T_j the run sums of the scalars;
+    a run's repeated base also drives the doubling and cancelling cases of
+    the bucket additions."""
+    n, block = 1 << log_n, 64
+    g1 = O.g1_generator()
+    k = rand_fr(1200 + log_n, n // block)
+    bases = np.repeat(N.fixed_base_mul(g1, k), block, axis=0)
+    m = N.Msm(bases)
+    del bases
+    kk = O.fr_ints(k)
+    s = rand_fr(1300 + log_n, n)
+    for variant in ("uniform", "witness"):
+        if variant == "witness":
+            s = witness_like(s, 1400 + log_n)
+        t = block_sums(s, block)
+        check_block_sums(s, t, block)
+        tot = sum(a * b for a, b in zip(kk, t)) % O.R
+        want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
+        assert np.array_equal(m.run(s), want), (log_n, variant)
+    m.close()
+
+
+def test_msm_g2_2_18_closed_form():
+    """G2 MSM at 2^18 terms with uniform scalars, closed form as for G1."""
+    n, block = 1 << 18, 64
+    g2 = O.g2_generator()
+    k = rand_fr(1500, n // block)
+    bases = np.repeat(N.fixed_base_mul(g2, k, g2=True), block, axis=0)
+    s = rand_fr(1501, n)
+    t = block_sums(s, block)
+    check_block_sums(s, t, block)
+    tot = sum(a * b for a, b in zip(O.fr_ints(k), t)) % O.R
+    want = O.g2_to_affine(O.g2_mul(g2, O.fr_array([tot])))[0]
+    assert np.array_equal(N.msm(bases, s, g2=True), want)
+
+
+def test_spmv_2_20_rows():
+    """CSR SpMV at 2^20 rows, 3 non-zeros per row on average (Poisson), uniform
+    Fr coefficients, against the oracle."""
+    rows = cols = 1 << 20
+    rng = np.random.default_rng(13)
+    per_row = rng.poisson(3, rows)
+    row_ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    nnz = int(row_ptr[-1])
+    col_idx = rng.integers(0, cols, nnz).astype(np.uint32)
+    vals = rand_fr(1601, nnz)
+    z = rand_fr(1602, cols)
+    got = N.spmv(rows, cols, row_ptr, col_idx, vals, z)
+    assert np.array_equal(got, O.spmv(rows, cols, row_ptr, col_idx, vals, z))
