@@ -42,6 +42,8 @@ HBM_PEAK_GBS = 8000.0                   # MI355X HBM3E, published
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9   # 256 CUs x 64 lanes x 2.4 GHz; v_mad_u64_u32 issues at full rate
 MADS_PER_FQ_MUL = 136                   # 2*8^2 + 8 for an 8x32-bit Montgomery product
 FQ_MUL_PEAK = VALU_PEAK_LANE_OPS / MADS_PER_FQ_MUL
+ONE_MONT = np.array([((1 << 256) % circuits.R >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(4)],
+                    dtype=np.uint64)    # 1 in Montgomery form (R = 2^256 mod r)
 STAGES = ["spmv", "ntt", "quotient", "msm_sort", "msm_accumulate", "msm_fold", "finish_host"]
 
 
@@ -146,6 +148,92 @@ def cpu_baseline(circ, pk, witness_np, h_host, budget_s: float) -> dict:
     }
 
 
+def operators_leg(reps: int) -> list:
+    """Standalone operators at the sizes of the reference's operator
+    configuration, inputs resident in HBM: forward Fr NTT at 2^16, 2^20 and
+    2^24 (nkr_fr_ntt_dev, in place); CSR SpMV over 2^20 rows with 3 non-zeros
+    per row on average (nkr_spmv_run_dev, matrix uploaded beforehand); G1 MSM at 2^16, 2^20 and 2^24 with uniform
+    and witness-like (half 0/1) scalars and G2 MSM at 2^18 with uniform
+    scalars (nkr_msm_run_dev, window-shifted bases prepared beforehand, the
+    host's final combination included).  Bases are distinct multiples of
+    points from the GPU setup of a small circuit.  Time per call = wall time
+    of `reps` back-to-back calls between device synchronisations."""
+    lib = N.lib()
+    rows = []
+
+    def to_dev(a):
+        ptr = ctypes.c_void_p()
+        N._check(lib.nkr_dev_alloc(a.nbytes, ctypes.byref(ptr)))
+        N._check(lib.nkr_memcpy_htod(ptr, N._p(a), a.nbytes))
+        return ptr
+
+    def timed(fn):
+        fn()
+        N._check(lib.nkr_device_synchronize())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        N._check(lib.nkr_device_synchronize())
+        return 1e3 * (time.perf_counter() - t0) / reps
+
+    for log_n in (16, 20, 24):
+        n = 1 << log_n
+        buf = to_dev(circuits.random_scalars(n, SEED + log_n))
+        ms = timed(lambda: N._check(lib.nkr_fr_ntt_dev(buf, log_n, N.FFT, None)))
+        N._check(lib.nkr_dev_free(buf))
+        products = n // 2 * log_n
+        rows.append({"op": "fr_ntt_forward", "log_n": log_n, "ms": round(ms, 4),
+                     "m_elements_per_s": round(n / ms / 1e3, 1),
+                     "g_products_per_s": round(products / ms / 1e6, 2),
+                     "valu_frac": round(products / ms * 1e3 / FQ_MUL_PEAK, 4)})
+        log(f"operators: NTT 2^{log_n} {ms:.3f} ms")
+
+    rows_n = 1 << 20
+    rng = np.random.default_rng(SEED)
+    per_row = rng.poisson(3, rows_n)
+    row_ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    nnz = int(row_ptr[-1])
+    sp = N.Spmv(rows_n, rows_n, row_ptr, rng.integers(0, rows_n, nnz).astype(np.uint32),
+                circuits.random_scalars(nnz, SEED ^ 0x5B))
+    z = to_dev(circuits.random_scalars(rows_n, SEED ^ 0x5C))
+    y = to_dev(np.zeros((rows_n, 4), dtype=np.uint64))
+    ms = timed(lambda: sp.run_dev(z.value, y.value))
+    sp.close()
+    N._check(lib.nkr_dev_free(z))
+    N._check(lib.nkr_dev_free(y))
+    alg = 36 * nnz + 36 * rows_n + 32 * rows_n  # values + columns, offsets + output, gathered z (once)
+    rows.append({"op": "spmv", "rows": rows_n, "nnz": nnz, "ms": round(ms, 4),
+                 "alg_gb_per_s": round(alg / ms / 1e6, 1), "hbm_frac": round(alg / ms / 1e6 / HBM_PEAK_GBS, 4)})
+    log(f"operators: SpMV 2^20 rows, {nnz} non-zeros {ms:.3f} ms")
+
+    small = circuits.matmul(2, SEED, wide=True)
+    keys = N.setup(small, circuits.random_scalars(5, SEED ^ 0x0B5))
+    for g2, sizes in ((False, (16, 20, 24)), (True, (18,))):
+        base = keys.beta_g2 if g2 else keys.alpha_g1
+        for log_n in sizes:
+            n = 1 << log_n
+            bases = N.fixed_base_mul(base, circuits.random_scalars(n, SEED ^ (0xBA5E + log_n)), g2=g2)
+            m = N.Msm(bases, g2=g2)
+            del bases
+            s = circuits.random_scalars(n, SEED ^ (0x5CA1 + log_n))
+            variants = ["uniform"] if g2 else ["uniform", "witness"]
+            for variant in variants:
+                if variant == "witness":
+                    rng = np.random.default_rng(SEED + log_n)
+                    pick = rng.random(n) < 0.5
+                    s[pick] = 0
+                    s[pick & (rng.random(n) < 0.5)] = ONE_MONT
+                buf = to_dev(s)
+                ms = timed(lambda: m.run_dev(buf.value))
+                N._check(lib.nkr_dev_free(buf))
+                rows.append({"op": "g2_msm" if g2 else "g1_msm", "log_n": log_n, "scalars": variant,
+                             "window_bits": m.window_bits, "ms": round(ms, 4),
+                             "m_terms_per_s": round(n / ms / 1e3, 1)})
+                log(f"operators: {'G2' if g2 else 'G1'} MSM 2^{log_n} {variant} {ms:.3f} ms")
+            m.close()
+    return rows
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,9 +255,17 @@ def main() -> int:
     ap.add_argument("--pmc-workload", action="store_true",
                     help="only run the counter-collection workload: three 256 MiB device copies (the "
                          "calibration kernel), then two proves of the 2^20 circuit")
+    ap.add_argument("--operators", action="store_true",
+                    help="only time the standalone operators (NTT, G1/G2 MSM) at the operator-configuration "
+                         "sizes and print them as one JSON line")
+    ap.add_argument("--op-reps", type=int, default=10)
     args = ap.parse_args()
     if args.pmc_workload:
         return pmc_workload(args.size, args.density)
+    if args.operators:
+        print(json.dumps({"operators": operators_leg(args.op_reps), "data": "synthetic (seeded)",
+                          "inputs": "resident in HBM"}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -290,6 +290,18 @@ int nkr_msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap);
 int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,
              uint64_t nnz, const void *z, void *y, int device);
 
+/* A CSR matrix uploaded once, then y = M * z over device vectors: the
+ * reference's spMVMContext (backends/hip/include/spmvm/spmvm.hip:24-72), whose
+ * constructor uploads the matrices and whose spmvm multiplies device
+ * vectors.  The host arrays are checked (offsets, columns) and copied at
+ * create; run_dev is asynchronous on `stream` (NULL: the null stream) and
+ * reads z_dev[cols], writes y_dev[rows]. */
+typedef struct nkr_spmv_ctx nkr_spmv_ctx;
+int nkr_spmv_create(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx,
+                    const void *values, uint64_t nnz, int device, nkr_spmv_ctx **out);
+int nkr_spmv_run_dev(nkr_spmv_ctx *ctx, const void *z_dev, void *y_dev, void *stream);
+int nkr_spmv_destroy(nkr_spmv_ctx *ctx);
+
 /* MSM with prepared bases (the reference's MSMContext: load_bases, then msm).
  * bases: n reference-layout points (any Z; infinity allowed).  c = 0 picks
  * the window width. */

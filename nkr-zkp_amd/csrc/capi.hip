@@ -537,15 +537,23 @@ int nkr_ntt_set_shape(uint32_t lds_log, uint32_t radix_log, uint32_t split_polic
     });
 }
 
+// The CSR checks every SpMV entry point makes before touching the device.
+static void check_csr(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx,
+                      const void *values, uint64_t nnz)
+{
+    NKR_REQUIRE(row_ptr && (nnz == 0 || (col_idx && values)), "null argument");
+    NKR_REQUIRE(rows < (1ull << 32) && nnz < (1ull << 32) && cols < (1ull << 32), "matrix too large");
+    NKR_REQUIRE(row_ptr[0] == 0 && row_ptr[rows] == nnz, "invalid CSR offsets");
+    for (uint64_t r = 0; r < rows; ++r) NKR_REQUIRE(row_ptr[r + 1] >= row_ptr[r], "non-monotonic CSR offsets");
+    for (uint64_t j = 0; j < nnz; ++j) NKR_REQUIRE(col_idx[j] < cols, "out-of-range column");
+}
+
 int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx, const void *values,
              uint64_t nnz, const void *z, void *y, int device)
 {
     return guarded([&] {
-        NKR_REQUIRE(row_ptr && z && y && (nnz == 0 || (col_idx && values)), "null argument");
-        NKR_REQUIRE(rows < (1ull << 32) && nnz < (1ull << 32) && cols < (1ull << 32), "matrix too large");
-        NKR_REQUIRE(row_ptr[0] == 0 && row_ptr[rows] == nnz, "invalid CSR offsets");
-        for (uint64_t r = 0; r < rows; ++r) NKR_REQUIRE(row_ptr[r + 1] >= row_ptr[r], "non-monotonic CSR offsets");
-        for (uint64_t j = 0; j < nnz; ++j) NKR_REQUIRE(col_idx[j] < cols, "out-of-range column");
+        NKR_REQUIRE(z && y, "null argument");
+        check_csr(rows, cols, row_ptr, col_idx, values, nnz);
         set_device(device);
         DeviceBuffer<uint32_t> rp(rows + 1), ci(std::max<uint64_t>(nnz, 1));
         DeviceBuffer<fr_t> v(std::max<uint64_t>(nnz, 1)), dz(std::max<uint64_t>(cols, 1)), dy(std::max<uint64_t>(rows, 1));
@@ -561,6 +569,58 @@ int nkr_spmv(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32
         m.long_rows = lr.get();
         spmv(m, dz.get(), dy.get(), nullptr);
         if (rows) NKR_HIP(hipMemcpy(y, dy.get(), rows * sizeof(fr_t), hipMemcpyDeviceToHost));
+    });
+}
+
+struct nkr_spmv_ctx {
+    int device = 0;
+    nkr::DeviceBuffer<uint32_t> row_ptr, col_idx, long_rows;
+    nkr::DeviceBuffer<nkr::fr_t> values;
+    nkr::CsrView view{};
+    uint64_t cols = 0;
+};
+
+int nkr_spmv_create(uint64_t rows, uint64_t cols, const uint32_t *row_ptr, const uint32_t *col_idx,
+                    const void *values, uint64_t nnz, int device, nkr_spmv_ctx **out)
+{
+    return guarded([&] {
+        NKR_REQUIRE(out, "null argument");
+        *out = nullptr;
+        check_csr(rows, cols, row_ptr, col_idx, values, nnz);
+        set_device(device);
+        auto m = std::make_unique<nkr_spmv_ctx>();
+        m->device = device;
+        m->cols = cols;
+        m->row_ptr.allocate(rows + 1);
+        m->col_idx.allocate(std::max<uint64_t>(nnz, 1));
+        m->values.allocate(std::max<uint64_t>(nnz, 1));
+        NKR_HIP(hipMemcpy(m->row_ptr.get(), row_ptr, (rows + 1) * 4, hipMemcpyHostToDevice));
+        if (nnz) {
+            NKR_HIP(hipMemcpy(m->col_idx.get(), col_idx, nnz * 4, hipMemcpyHostToDevice));
+            NKR_HIP(hipMemcpy(m->values.get(), values, nnz * sizeof(fr_t), hipMemcpyHostToDevice));
+        }
+        m->view = CsrView{m->row_ptr.get(), m->col_idx.get(), m->values.get(), (uint32_t)rows, (uint32_t)nnz};
+        m->view.n_long = spmv_long_rows(row_ptr, (uint32_t)rows, m->long_rows);
+        m->view.long_rows = m->long_rows.get();
+        *out = m.release();
+    });
+}
+
+int nkr_spmv_run_dev(nkr_spmv_ctx *m, const void *z_dev, void *y_dev, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(m && z_dev && y_dev, "null argument");
+        NKR_HIP(hipSetDevice(m->device));
+        spmv(m->view, static_cast<const fr_t *>(z_dev), static_cast<fr_t *>(y_dev),
+             static_cast<hipStream_t>(stream));
+    });
+}
+
+int nkr_spmv_destroy(nkr_spmv_ctx *m)
+{
+    return guarded([&] {
+        if (m) NKR_HIP(hipSetDevice(m->device));
+        delete m;
     });
 }
 

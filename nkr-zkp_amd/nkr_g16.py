@@ -103,6 +103,9 @@ def lib() -> ctypes.CDLL:
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
             "nkr_spmv": [u64, u64, vp, vp, vp, u64, vp, vp, i32],
+            "nkr_spmv_create": [u64, u64, vp, vp, vp, u64, i32, ctypes.POINTER(vp)],
+            "nkr_spmv_run_dev": [vp, vp, vp, vp],
+            "nkr_spmv_destroy": [vp],
             "nkr_g1_msm_create": [vp, u64, u32, i32, ctypes.POINTER(vp)],
             "nkr_g2_msm_create": [vp, u64, u32, i32, ctypes.POINTER(vp)],
             "nkr_msm_run": [vp, vp, vp],
@@ -214,6 +217,35 @@ def spmv(rows, cols, row_ptr, col_idx, values, z, device: int = 0) -> np.ndarray
     _check(lib().nkr_spmv(rows, cols, _p(row_ptr), _p(col_idx) if col_idx.size else None,
                           _p(values) if values.size else None, values.shape[0], _p(z), _p(y), device))
     return y
+
+
+class Spmv:
+    """One CSR matrix uploaded once (spMVMContext), multiplied with device vectors."""
+
+    def __init__(self, rows, cols, row_ptr, col_idx, values, device: int = 0):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        values = _words(values, 4)
+        self.rows, self.cols = rows, cols
+        h = ctypes.c_void_p()
+        _check(lib().nkr_spmv_create(rows, cols, _p(row_ptr), _p(col_idx) if col_idx.size else None,
+                                     _p(values) if values.size else None, values.shape[0], device,
+                                     ctypes.byref(h)))
+        self.h = h
+
+    def run_dev(self, z_dev: int, y_dev: int, stream: int | None = None) -> None:
+        _check(lib().nkr_spmv_run_dev(self.h, z_dev, y_dev, stream))
+
+    def close(self) -> None:
+        if self.h:
+            lib().nkr_spmv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Msm:

@@ -1,5 +1,7 @@
 """Parity of the HIP operators against the CPU oracle and the golden vectors,
 called through the C ABI.  Integer field work: the bar is bit-exact equality."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -302,3 +304,38 @@ def test_spmv_2_20_rows():
     z = rand_fr(1602, cols)
     got = N.spmv(rows, cols, row_ptr, col_idx, vals, z)
     assert np.array_equal(got, O.spmv(rows, cols, row_ptr, col_idx, vals, z))
+
+
+def test_spmv_context_device_vectors():
+    """nkr_spmv_create / nkr_spmv_run_dev (spMVMContext): the matrix uploaded
+    once, two different device vectors, each equal to the oracle; bad CSR
+    input is rejected at create."""
+    rows, cols = 5000, 3000
+    rng = np.random.default_rng(17)
+    per_row = rng.integers(0, 6, rows)
+    per_row[::101] = 40  # wave-per-row path
+    row_ptr = np.concatenate([[0], np.cumsum(per_row)]).astype(np.uint32)
+    nnz = int(row_ptr[-1])
+    col_idx = rng.integers(0, cols, nnz).astype(np.uint32)
+    vals = rand_fr(1701, nnz)
+    sp = N.Spmv(rows, cols, row_ptr, col_idx, vals)
+    lib = N.lib()
+    ydev = ctypes.c_void_p()
+    N._check(lib.nkr_dev_alloc(rows * 32, ctypes.byref(ydev)))
+    for seed in (1702, 1703):
+        z = rand_fr(seed, cols)
+        zdev = ctypes.c_void_p()
+        N._check(lib.nkr_dev_alloc(z.nbytes, ctypes.byref(zdev)))
+        N._check(lib.nkr_memcpy_htod(zdev, N._p(z), z.nbytes))
+        sp.run_dev(zdev.value, ydev.value)
+        N._check(lib.nkr_device_synchronize())
+        y = np.zeros((rows, 4), dtype=np.uint64)
+        N._check(lib.nkr_memcpy_dtoh(N._p(y), ydev, y.nbytes))
+        N._check(lib.nkr_dev_free(zdev))
+        assert np.array_equal(y, O.spmv(rows, cols, row_ptr, col_idx, vals, z)), seed
+    N._check(lib.nkr_dev_free(ydev))
+    sp.close()
+    bad = col_idx.copy()
+    bad[7] = cols
+    with pytest.raises(N.NkrError, match="out-of-range column"):
+        N.Spmv(rows, cols, row_ptr, bad, vals)
