@@ -261,8 +261,12 @@ __device__ void schedule_scan(const uint32_t *__restrict__ count, uint32_t B, ui
 {
     const uint32_t entries = block_scan(count, B, 0, 1, 0, bucket_start);
     const uint32_t q = entries >> target_log;
+    // Longer pieces also while an average bucket would be left with more
+    // than kMsmCompleteMax partials: small MSMs (few entries against the
+    // piece target) then complete without fold levels.  Longer pieces only
+    // lower the piece count, so the host's max_pieces bound still holds.
     uint32_t piece = piece_lo;
-    while (piece < piece_hi && 2 * piece <= q) piece <<= 1;
+    while (piece < piece_hi && (2 * piece <= q || (uint64_t)piece * B * kMsmCompleteMax < entries)) piece <<= 1;
     const uint32_t pieces = block_scan(count, B, 2, piece, 0, piece_start, bucket_start + B + 2);
     if (threadIdx.x == 0) {
         bucket_start[B + 1] = pieces;
