@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "nkr-zkp_amd"))
 import numpy as np  # noqa: E402
 
 import circuits  # noqa: E402
+import autotune  # noqa: E402
 import nkr_g16 as N  # noqa: E402
 
 SEED = 0x4E4B52
@@ -241,6 +242,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=100, help="matmul dimension s (100 -> 2^20 domain)")
     ap.add_argument("--density", default="sparse")
+    ap.add_argument("--circuit", choices=["matmul", "poseidon"], default="matmul",
+                    help="matmul (the default workload) or the Poseidon-style batch: --hashes independent "
+                         "width-3 hashes, folded rounds (~240 constraints each; 4096 -> 2^20 domain)")
+    ap.add_argument("--hashes", type=int, default=4096)
+    ap.add_argument("--tuned", action="store_true",
+                    help="build the context with the shipped MI355X tuning table's entry for this domain "
+                         "size and witness density (autotune.tuning_for); the default uses library defaults")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain mix")
@@ -273,14 +281,23 @@ def main() -> int:
                                    "this build (proofs shard across GPUs as independent work lists)"}))
         return 2
 
-    log(f"generating matmul {args.size} ({args.density}) seed={SEED:#x}")
-    circ = circuits.matmul(args.size, SEED, density=args.density, wide=True)
+    if args.circuit == "poseidon":
+        log(f"generating {args.hashes} folded Poseidon-style hashes seed={SEED:#x} (about 80 s for 4096)")
+        circ = circuits.poseidon(args.hashes, SEED, folded=True)
+        workload = f"groth16_prove_poseidon{args.hashes}_folded"
+    else:
+        log(f"generating matmul {args.size} ({args.density}) seed={SEED:#x}")
+        circ = circuits.matmul(args.size, SEED, density=args.density, wide=True)
+        workload = f"groth16_prove_matmul{args.size}_{args.density}"
     log(f"rows={circ.rows} cols={circ.cols} nnz=({circ.a[1].size},{circ.b[1].size},{circ.c[1].size})")
     t0 = time.perf_counter()
     pk = N.setup(circ, circuits.random_scalars(5, SEED ^ 0x5E7))
     log(f"proving and verification keys generated on the GPU ({time.perf_counter() - t0:.2f}s)")
     t0 = time.perf_counter()
-    prover = N.Prover(circ, pk)
+    tuning = autotune.tuning_for(circ, circ.witness) if args.tuned else None
+    if args.tuned:
+        log(f"tuning from the shipped table: {tuning or 'library defaults'}")
+    prover = N.Prover(circ, pk, tuning=tuning)
     prep_s = time.perf_counter() - t0
     log(f"prepare {prep_s:.2f}s, device bytes {prover.device_bytes() / 2**30:.2f} GiB")
 
@@ -343,10 +360,13 @@ def main() -> int:
     alg_fq_mul = 3 * e_w * 10 + e_w * 30 + e_h * 10
     achieved_gbs = alg_bytes / (acc_ms * 1e-3) / 1e9
     achieved_mul = alg_fq_mul / (acc_ms * 1e-3)
-    cw, ch = min(15, _msm_window(circ.cols)), _msm_window(circ.rows)  # witness windows stop at 15 bits
+    # witness windows stop at 15 bits; a tuned context may set either width
+    cw = (tuning or {}).get("witness_window_bits") or min(15, _msm_window(circ.cols))
+    ch = (tuning or {}).get("h_window_bits") or _msm_window(circ.rows)
     # measured HBM traffic of the same launches (rocprofv3 counters of this
     # exact workload), per launch like `achieved`
-    traffic_prove = measured_traffic() if (args.size, args.density) == (100, "sparse") else None
+    traffic_prove = measured_traffic() if (args.circuit, args.size, args.density) == ("matmul", 100, "sparse") \
+        else None
     traffic = int(traffic_prove / launches) if traffic_prove and launches else None
     result = {
         "metric": "Groth16 prove latency @2^20 constraints; batched proofs/s vs plain (bit-exact)",
@@ -360,9 +380,10 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32 limbs (BN254 Fr/Fq Montgomery, exact)",
-        "data": "synthetic (seeded matmul R1CS and witness; keys from the GPU setup with seeded trapdoors)",
-        "config": {"workload": f"groth16_prove_matmul{args.size}_{args.density}_domain_2^{circ.rows.bit_length() - 1}",
+        "data": f"synthetic (seeded {args.circuit} R1CS and witness; keys from the GPU setup with seeded trapdoors)",
+        "config": {"workload": f"{workload}_domain_2^{circ.rows.bit_length() - 1}",
                    "rows": circ.rows, "cols": circ.cols, "path": "plain (one proof per call)",
+                   "tuning": tuning or {},
                    "msm_window_bits": {"witness": cw, "h": ch}},
         "stages_ms": stages,
         "blind_ms_host_stateless": round(blind_ms, 3),
