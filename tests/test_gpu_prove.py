@@ -5,6 +5,7 @@ with fixed r, s are bit-identical in the reference's affine form."""
 import numpy as np
 import pytest
 
+import circuits
 import gen_golden as GG
 import goldens
 import nkr_g16 as N
@@ -120,3 +121,16 @@ def test_prove_rejects_bad_inputs():
     circ.rows = e.rows - 1
     with pytest.raises(N.NkrError, match="power of two"):
         N.Prover(circ, e)
+
+
+@pytest.mark.parametrize("count,folded", [(6, False), (16, True)])
+def test_prove_poseidon_vs_oracle(count, folded):
+    """Poseidon-style hash batches (the second circuit family of the large-proof
+    configuration, at small counts; dense witnesses with uniform preimages):
+    the raw proof equals the oracle's."""
+    circ = circuits.poseidon(count, 0x9A + count, folded=folded)
+    pk = synthetic_pk(circ.rows, circ.cols, 93 + count)
+    pv = N.Prover(circ, pk)
+    raw = pv.prove(circ.witness)
+    assert np.array_equal(raw, affine_raw(O.prove(circ, pk, circ.witness)))
+    pv.close()
