@@ -17,6 +17,7 @@ Results are kept per (domain size, witness density) in a JSON table that
 measured on MI355X ships as `tuning_mi355x.json` beside this file.
 
     python autotune.py --size 100 --density sparse --out tuning.json
+    python autotune.py --circuit poseidon --hashes 4096 --out tuning.json
 """
 from __future__ import annotations
 
@@ -209,14 +210,19 @@ def tuning_for(circ, witness) -> dict | None:
 def main() -> int:
     import circuits
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--circuit", choices=["matmul", "poseidon"], default="matmul")
     ap.add_argument("--size", type=int, default=100, help="matmul dimension s")
     ap.add_argument("--density", default="sparse")
+    ap.add_argument("--hashes", type=int, default=4096, help="Poseidon-style hashes (folded rounds)")
     ap.add_argument("--proves", type=int, default=10)
     ap.add_argument("--seed", type=lambda x: int(x, 0), default=0x4E4B52)
     ap.add_argument("--min-gain", type=float, default=0.01)
     ap.add_argument("--out", default="tuning.json")
     args = ap.parse_args()
-    circ = circuits.matmul(args.size, args.seed, density=args.density, wide=True)
+    if args.circuit == "poseidon":
+        circ = circuits.poseidon(args.hashes, args.seed, folded=True)
+    else:
+        circ = circuits.matmul(args.size, args.seed, density=args.density, wide=True)
     pk = N.setup(circ, circuits.random_scalars(5, args.seed ^ 0x5E7))
     rs = circuits.random_scalars(2, args.seed ^ 0x72)
     best, _ = autotune(circ, pk, circ.witness, rs[0], rs[1], proves=args.proves, min_gain=args.min_gain,
