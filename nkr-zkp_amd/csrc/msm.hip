@@ -464,7 +464,10 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
 // Fold level L: buckets with more than kMsmCompleteMax partials get
 // ceil(count / kMsmFoldIn) new partials written at pool[level_base[L] ...].
 template <class F>
-__global__ __launch_bounds__(1024) void k_fold_scan(const JobDev<F> *__restrict__ jobs)
+// 256 threads (one wave per SIMD): on the side stream this launch runs beside
+// a resident-sized accumulation on the main stream, and a smaller workgroup
+// needs less free room on a CU to start.
+__global__ __launch_bounds__(256) void k_fold_scan(const JobDev<F> *__restrict__ jobs)
 {
     const JobDev<F> &jb = jobs[blockIdx.x];
     if (*jb.piece_max <= kMsmCompleteMax) return;  // no skewed bucket: nothing to fold
@@ -1070,7 +1073,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     }
     NKR_REQUIRE(folds <= fold_levels_, "MSM bucket exceeds the partial-sum capacity");
     for (uint32_t L = 0; L < folds; ++L) {
-        k_fold_scan<F><<<J, 1024, 0, stream>>>(jd);
+        k_fold_scan<F><<<J, 256, 0, stream>>>(jd);
         NKR_LAUNCH_CHECK();
         k_fold<F><<<dim3(div_up(fold_cap_[L], 256), J), 256, 0, stream>>>(jd, L);
         NKR_LAUNCH_CHECK();
