@@ -179,10 +179,13 @@ int nkr_g16_server_destroy(nkr_g16_server *server);
 
 /* Stage timing with HIP events on the context's stream.  While enabled,
  * every prove adds its per-stage device time (ms) to running totals:
- *   0 SpMV, 1 NTT passes (7 transforms), 2 quotient pointwise, 3 MSM digit
- *   sort (2 schedules), 4 MSM bucket accumulation (3 launches: the witness G1
- *   batch, the G2 MSM, the H MSM), 5 MSM folding
- *   and bit planes, 6 plane download + host finish.
+ *   0 SpMV, 1 NTT passes (6 transforms: 3 iFFT, 3 cosetFFT; the H query is
+ *   prepared in the coset-evaluation basis, so no icosetFFT runs), 2 quotient
+ *   pointwise, 3 MSM digit sort (2 schedules: witness and quotient),
+ *   4 MSM bucket accumulation (2 launches: one G1 batch of ar, bs1, zk and
+ *   H; the G2 MSM bs2), 5 MSM folding, completion and bit planes, 6 plane
+ *   download + host finish.  Stages on the two streams overlap, so the
+ *   totals do not add up to the proof's latency.
  * counters[0] = proves timed, counters[1] = accumulation launches,
  * counters[2] = bucket entries (non-zero signed digits) of the witness
  * schedule, summed over proves (each is used by 3 G1 MSMs and the G2 MSM),

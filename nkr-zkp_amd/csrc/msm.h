@@ -38,26 +38,10 @@
 #include <vector>
 
 #include "curve.h"
+#include "msm_plan.h"
 #include "runtime.h"
 
 namespace nkr {
-
-// Largest accumulation piece, fold fan-in and the completion limit.  Small
-// MSMs use shorter pieces (down to kMsmPieceMin) so that the serial chain of
-// one thread, not the amount of work, stops dominating their latency.
-constexpr uint32_t kMsmPiece = 64;
-constexpr uint32_t kMsmPieceMin = 4;
-constexpr uint32_t kMsmFoldIn = 4;
-constexpr uint32_t kMsmCompleteMax = 64;
-constexpr uint32_t kMaxFoldLevels = 12;
-// The piece length is chosen on the device once the number of non-zero
-// digits E is known: the largest power of two <= E / 2^target_log,
-// clamped to [piece_lo, piece_hi], so a schedule has about 2^target_log
-// pieces (default 2^19: every resident accumulation thread busy).
-constexpr uint32_t kMsmTargetPiecesLog = 19;
-// Largest bucket count (c <= 16): the histograms and bucket scans hold B
-// counts in one workgroup's LDS (128 KiB of gfx950's 160 KiB).
-constexpr uint32_t kMsmMaxBuckets = 1u << 15;
 
 // Process-wide MSM tuning knobs: the target piece count (2^target_log, 10 ..
 // 22) and the most lanes that complete one bucket (1 .. 16, a power of two).
@@ -65,32 +49,6 @@ constexpr uint32_t kMsmMaxBuckets = 1u << 15;
 void msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap);
 uint32_t msm_target_pieces_log();
 uint32_t msm_complete_lanes_cap();
-
-struct MsmPlan {
-    uint32_t n = 0;         // number of bases / scalars
-    uint32_t c = 0;         // window width in bits
-    uint32_t W = 0;         // windows (signed digits), c*W >= 255
-    uint32_t B = 0;         // buckets 1..B, B = 2^(c-1)
-    uint32_t piece_lo = 0;  // bounds of the accumulation piece length
-    uint32_t piece_hi = 0;
-    static MsmPlan choose(uint32_t n, uint32_t c_override = 0);
-    uint64_t max_entries() const { return (uint64_t)n * W; }
-    // Upper bound of level-0 pieces for any scalars when the device targets
-    // 2^target_log pieces: the piece p it picks is piece_hi, or p > E/2^(target_log+1).
-    uint64_t max_pieces(uint32_t target_log) const
-    {
-        const uint64_t by_lo = max_entries() / piece_lo;
-        const uint64_t by_target = std::max<uint64_t>(2ull << target_log, max_entries() / piece_hi);
-        return std::min(by_lo, by_target) + B + 1;
-    }
-    // The same plan with a fixed piece length.
-    MsmPlan with_piece(uint32_t piece) const
-    {
-        MsmPlan p = *this;
-        p.piece_lo = p.piece_hi = piece;
-        return p;
-    }
-};
 
 // Device bytes of one schedule (MsmSchedule) and of one MSM job's partial
 // sums and per-bucket arrays (MsmBatch) for a plan.
@@ -151,12 +109,17 @@ public:
     void set_target_log(uint32_t target_log) { target_override_ = target_log; }
     uint64_t alt_max_pieces() const
     {
+        NKR_REQUIRE(alt_piece_lo_ > 0, "an MSM job reads the second piece split of a schedule that has none");
         MsmPlan p = plan_;
         p.piece_lo = alt_piece_lo_;
         p.piece_hi = std::max(alt_piece_lo_, p.piece_hi);
         return p.max_pieces(target_log_);
     }
-    const uint32_t *alt_totals_host() const { return totals_host() + 4; }
+    const uint32_t *alt_totals_host() const
+    {
+        NKR_REQUIRE(alt_piece_lo_ > 0, "an MSM job reads the second piece split of a schedule that has none");
+        return totals_host() + 4;
+    }
 
     MsmPlan plan_;
     uint32_t blocks_ = 0;

@@ -738,40 +738,6 @@ void msm_set_shape(uint32_t target_pieces_log, uint32_t complete_lanes_cap)
 uint32_t msm_target_pieces_log() { return g_target_log; }
 uint32_t msm_complete_lanes_cap() { return g_lanes_cap; }
 
-MsmPlan MsmPlan::choose(uint32_t n, uint32_t c_override)
-{
-    NKR_REQUIRE(n > 0, "MSM needs at least one term");
-    MsmPlan p;
-    p.n = n;
-    uint32_t c = c_override;
-    if (c == 0) {
-        const unsigned lg = ceil_log2(n);
-        // ~3/4 log2 n; 16 bits from 2^20 terms up (16 windows instead of 17,
-        // its 128 KiB bucket histogram still fits one workgroup's LDS).
-        c = lg >= 20 ? 16u : std::max(4u, std::min(15u, (lg * 3 + 2) / 4));
-        // Scalars have 254 bits; a top window with only a few of them sends a
-        // whole window's entries to a handful of buckets (long serial chains),
-        // so move to a neighbouring width whose top window keeps >= 4 bits.
-        auto top_bits = [](uint32_t cc) { return 254 - cc * ((255 + cc - 1) / cc - 1); };
-        if (top_bits(c) < 4) {
-            if (c < 16 && top_bits(c + 1) >= 4)
-                c += 1;
-            else if (c > 4 && top_bits(c - 1) >= 4)
-                c -= 1;
-        }
-    }
-    NKR_REQUIRE(c >= 2 && c <= 16, "MSM window must be 2..16 bits");
-    p.c = c;
-    p.W = (255 + c - 1) / c;
-    p.B = 1u << (c - 1);
-    // The piece length follows the actual number of non-zero digits (chosen
-    // on the device): sparse witnesses get short serial chains.
-    p.piece_lo = kMsmPieceMin;
-    p.piece_hi = kMsmPiece;
-    NKR_REQUIRE((uint64_t)p.W * n < (1ull << 31), "MSM too large for 31-bit entry indices");
-    return p;
-}
-
 uint32_t MsmSchedule::sort_blocks(uint32_t n)
 {
     return std::max<uint32_t>(1, std::min<uint32_t>(kSortBlocksTarget, div_up(n, kSortMinPerBlock)));
@@ -786,12 +752,7 @@ uint64_t msm_schedule_bytes(const MsmPlan &p)
 template <class F>
 uint64_t msm_job_bytes(const MsmPlan &p)
 {
-    uint64_t cap = p.max_pieces(msm_target_pieces_log()), cnt = cap, total = cap;
-    while (cnt > kMsmCompleteMax) {
-        cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
-        cap = cap / kMsmFoldIn + p.B + 1;
-        total += cap;
-    }
+    const uint64_t total = msm_partial_capacity(p, p.max_pieces(msm_target_pieces_log()), nullptr, nullptr);
     return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12) + p.c * sizeof(Xyzz<F>);
 }
 
@@ -966,19 +927,14 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
                     "MSM bases and schedule disagree");
         const uint64_t pieces = job_max_pieces(jobs[j]);
         thread_base_[j + 1] = thread_base_[j] + pieces;
-        uint64_t cap = pieces, cnt = pieces, base = pieces;
+        uint64_t lb[kMaxFoldLevels], lc[kMaxFoldLevels];
         uint32_t levels = 0;
-        while (cnt > kMsmCompleteMax) {
-            NKR_REQUIRE(levels < kMaxFoldLevels, "MSM needs too many fold levels");
-            cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
-            cap = cap / kMsmFoldIn + p.B + 1;
-            level_base[j][levels] = (uint32_t)base;
-            fold_cap_[levels] = std::max<uint64_t>(fold_cap_[levels], cap);
-            base += cap;
-            ++levels;
-        }
-        for (uint32_t L = levels; L < kMaxFoldLevels; ++L) level_base[j][L] = (uint32_t)base;
+        const uint64_t base = msm_partial_capacity(p, pieces, &levels, lb, lc);
         NKR_REQUIRE(base < (1ull << 32), "MSM partial pool exceeds 32-bit indexing");
+        for (uint32_t L = 0; L < kMaxFoldLevels; ++L) {
+            level_base[j][L] = (uint32_t)lb[L];
+            fold_cap_[L] = std::max<uint64_t>(fold_cap_[L], lc[L]);
+        }
         fold_levels_ = std::max(fold_levels_, levels);
         pool_off[j + 1] = pool_off[j] + base;
         b_off[j + 1] = b_off[j] + p.B;
@@ -1063,13 +1019,8 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     // so this wait overlaps the GPU's work instead of stalling it.
     uint32_t folds = 0;
     for (const auto &jb : jobs) {
-        uint64_t cnt = (jb.alt ? jb.sched->alt_totals_host() : jb.sched->totals_host())[2];
-        uint32_t levels = 0;
-        while (cnt > kMsmCompleteMax) {
-            cnt = (cnt + kMsmFoldIn - 1) / kMsmFoldIn;
-            ++levels;
-        }
-        folds = std::max(folds, levels);
+        const uint64_t cnt = (jb.alt ? jb.sched->alt_totals_host() : jb.sched->totals_host())[2];
+        folds = std::max(folds, msm_fold_levels(cnt));
     }
     NKR_REQUIRE(folds <= fold_levels_, "MSM bucket exceeds the partial-sum capacity");
     for (uint32_t L = 0; L < folds; ++L) {

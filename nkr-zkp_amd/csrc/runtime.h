@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "error.h"
+
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -15,20 +17,11 @@
 
 namespace nkr {
 
-struct Error : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
 #define NKR_HIP(call)                                                                         \
     do {                                                                                      \
         const hipError_t nkr_e_ = (call);                                                     \
         if (nkr_e_ != hipSuccess)                                                             \
             throw ::nkr::Error(std::string(#call) + " failed: " + hipGetErrorString(nkr_e_)); \
-    } while (0)
-
-#define NKR_REQUIRE(cond, msg)                                      \
-    do {                                                            \
-        if (!(cond)) throw ::nkr::Error(std::string(msg));          \
     } while (0)
 
 #define NKR_LAUNCH_CHECK() NKR_HIP(hipGetLastError())
@@ -220,16 +213,5 @@ private:
     size_t used_ = 0;
     std::vector<Rec> recs_;
 };
-
-inline unsigned ceil_log2(uint64_t n)
-{
-    unsigned r = 0;
-    while ((uint64_t(1) << r) < n) ++r;
-    return r;
-}
-
-inline bool is_pow2(uint64_t n) { return n != 0 && (n & (n - 1)) == 0; }
-
-inline uint32_t div_up(uint64_t a, uint64_t b) { return static_cast<uint32_t>((a + b - 1) / b); }
 
 }  // namespace nkr
