@@ -17,8 +17,18 @@ structure and draw every value from mt19937_64 streams of one seed:
   mt19937_64(seed ^ 0xB0B); zero positions for hybrid / sparse density come
   from a Fisher-Yates shuffle driven by mt19937_64(seed ^ 0x3A5C0 / 0x3A5C1),
   with ceil(count * density) non-zeros (density.hpp:55-64).
+
+The same generators exist in C++ (nkr-zkp_amd/host/gen.cpp, the nkr-g16-gen
+program, which writes the reference's .bin / .witness.bin files); generate()
+runs it and reads its files back, which is how the 2^20-constraint workloads
+are built quickly.  Both produce identical arrays.
 """
 from __future__ import annotations
+
+import os
+import struct
+import subprocess
+import tempfile
 
 import numpy as np
 
@@ -428,3 +438,75 @@ def random_scalars(n: int, seed: int) -> np.ndarray:
         out[filled:filled + take] = good[:take]
         filled += take
     return out
+
+
+# ------------------------------------------------- reference-format files
+HERE = os.path.dirname(os.path.abspath(__file__))
+GEN_BIN = os.path.join(HERE, "bin", "nkr-g16-gen")
+_MAGIC = b"LKZ1"
+
+
+class _Reader:
+    """BinaryArchive reader (common/include/binary_archive.hpp:17-97): magic
+    "LKZ1", u32 version 1, then values and u64-byte-length-prefixed vectors."""
+
+    def __init__(self, path: str):
+        self.data = np.fromfile(path, dtype=np.uint8)
+        if self.data.size < 8 or self.data[:4].tobytes() != _MAGIC or \
+                struct.unpack("<I", self.data[4:8].tobytes())[0] != 1:
+            raise ValueError(f"{path}: not a version-1 LKZ1 archive")
+        self.pos = 8
+
+    def u64(self) -> int:
+        v = struct.unpack("<Q", self.data[self.pos:self.pos + 8].tobytes())[0]
+        self.pos += 8
+        return v
+
+    def vector(self, dtype, width: int = 1) -> np.ndarray:
+        n = self.u64()
+        if self.pos + n > self.data.size:
+            raise ValueError("truncated archive")
+        a = self.data[self.pos:self.pos + n].view(dtype)
+        self.pos += n
+        return a.reshape(-1, width) if width > 1 else a
+
+
+def read_r1cs(path: str, witness_path: str | None = None) -> R1cs:
+    """A circuit file (io.cpp write_circuit layout: public input count,
+    columns, then row_ptr, col_idx and values of A, B and C) and, when given,
+    its witness file."""
+    r = _Reader(path)
+    public_inputs = r.u64()
+    cols = r.u64()
+    mats = []
+    for _ in range(3):
+        rp = r.vector(np.uint32)
+        ci = r.vector(np.uint32)
+        va = r.vector(np.uint64, 4)
+        mats.append((rp, ci, va))
+    rows = mats[0][0].size - 1
+    witness = None
+    if witness_path:
+        w = _Reader(witness_path)
+        witness = w.vector(np.uint64, 4)
+    return R1cs(rows, cols, public_inputs, mats[0], mats[1], mats[2], witness)
+
+
+def generate(case: str, param: int, seed: int, density: str = "full", wide: bool = True,
+             folded: bool = False) -> R1cs:
+    """Runs the C++ generator (nkr-g16-gen) and reads its files back.
+    case: matmul (param = size; wide=False for non-zero 32-bit entries),
+    fibonacci (param = steps) or poseidon (param = hash count)."""
+    if not os.path.exists(GEN_BIN):
+        raise RuntimeError(f"{GEN_BIN} is not built (make -C nkr-zkp_amd)")
+    args = [GEN_BIN, "--seed", str(seed), "--density", density]
+    if case == "matmul" and not wide:
+        args.append("--u32")
+    if case == "poseidon" and folded:
+        args.append("--folded")
+    with tempfile.TemporaryDirectory() as d:
+        res = subprocess.run(args + ["--path", d, case, str(param)], capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"nkr-g16-gen failed: {res.stderr.strip()}")
+        stem = [f for f in os.listdir(d) if f.endswith(".bin") and not f.endswith(".witness.bin")][0][:-4]
+        return read_r1cs(os.path.join(d, stem + ".bin"), os.path.join(d, stem + ".witness.bin"))

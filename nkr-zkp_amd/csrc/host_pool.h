@@ -28,7 +28,7 @@ public:
         for (int i = 0; i < count; ++i) slots_.push_back(std::make_unique<Slot>());
         for (auto &s : slots_) {
             Slot *p = s.get();
-            p->th = std::thread([this, p] { loop(*p); });
+            p->th = std::thread([p] { loop(*p); });
         }
     }
     ~HostHelpers()

@@ -3,66 +3,28 @@
 // it, runs Backend::setup on the GPU and writes the proving and verification
 // key files.  --seed fixes the trapdoors (the reference draws them from
 // std::random_device).
-#include <chrono>
-#include <cstdlib>
 #include <iostream>
 #include <string>
 
+#include "cli.hpp"
 #include "groth16.hpp"
 
 using namespace nkr::groth16;
-
-namespace {
-
-bool g_profile = false;
-
-class Scope {
-public:
-    explicit Scope(const char *label) : label_(label), t0_(std::chrono::steady_clock::now()) {}
-    ~Scope()
-    {
-        if (!g_profile) return;
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
-        std::cout << "[profile] " << label_ << " = " << ms << " ms\n";
-    }
-
-private:
-    const char *label_;
-    std::chrono::steady_clock::time_point t0_;
-};
-
-[[noreturn]] void usage(const char *program)
-{
-    std::cerr << "Usage: " << program
-              << " [--backend hip] [--path <dir>] [--seed <u64>] [--profile] <r1cs-file>\n";
-    std::exit(2);
-}
-
-}  // namespace
+using nkr::cli::Scope;
 
 int main(int argc, char **argv)
 {
-    std::filesystem::path directory = "app/data";
-    std::string backend_name = "hip", filename;
-    bool seeded = false;
-    uint64_t seed = 0;
-    for (int i = 1; i < argc; ++i) {
-        const std::string arg = argv[i];
-        if (arg == "--backend" && i + 1 < argc)
-            backend_name = argv[++i];
-        else if (arg == "--path" && i + 1 < argc)
-            directory = argv[++i];
-        else if (arg == "--seed" && i + 1 < argc) {
-            seed = std::stoull(argv[++i], nullptr, 0);
-            seeded = true;
-        } else if (arg == "--profile")
-            g_profile = true;
-        else if (arg == "--help" || arg == "-h" || (!arg.empty() && arg.front() == '-'))
-            usage(argv[0]);
-        else
-            filename = arg;
+    const std::string usage = "[--backend hip] [--path <dir>] [--seed <u64>] [--profile] <r1cs-file>";
+    const nkr::cli::Args args = nkr::cli::parse(
+        argc, argv, nkr::cli::kPath | nkr::cli::kBackend | nkr::cli::kSeed | nkr::cli::kProfile, usage);
+    if (args.positional.empty()) {
+        std::cerr << "Usage: " << argv[0] << ' ' << usage << '\n';
+        return 2;
     }
-    if (filename.empty()) usage(argv[0]);
+    const std::filesystem::path directory = args.directory;
+    const std::string backend_name = args.backend, filename = args.positional.back();
+    const bool seeded = args.seeded;
+    const uint64_t seed = args.seed;
 
     try {
         Scope total("groth16.setup.total");

@@ -3,61 +3,33 @@
 // input files (r1cs, witness for the public inputs, verification key, proof),
 // same messages and exit codes: 0 when the proof verifies, 1 when it does
 // not or on an error ("verify: <what>"), 2 on a usage error.
-#include <chrono>
-#include <cstdlib>
 #include <iostream>
 #include <string>
 
+#include "cli.hpp"
 #include "groth16.hpp"
 
 using namespace nkr::groth16;
-
-namespace {
-
-bool g_profile = false;
-
-class Scope {
-public:
-    explicit Scope(const char *label) : label_(label), t0_(std::chrono::steady_clock::now()) {}
-    ~Scope()
-    {
-        if (!g_profile) return;
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0_).count();
-        std::cout << "[profile] " << label_ << " = " << ms << " ms\n";
-    }
-
-private:
-    const char *label_;
-    std::chrono::steady_clock::time_point t0_;
-};
-
-[[noreturn]] void usage(const char *program)
-{
-    std::cerr << "Usage: " << program << " [--path <dir>] [--proof <proof-file>] [--profile] <r1cs-file>\n";
-    std::exit(2);
-}
-
-}  // namespace
+using nkr::cli::Scope;
 
 int main(int argc, char **argv)
 {
-    std::filesystem::path directory = "app/data";
     std::filesystem::path proof_input;
-    std::string filename;
-    for (int i = 1; i < argc; ++i) {
-        const std::string arg = argv[i];
-        if (arg == "--path" && i + 1 < argc)
-            directory = argv[++i];
-        else if (arg == "--proof" && i + 1 < argc)
-            proof_input = argv[++i];
-        else if (arg == "--profile")
-            g_profile = true;
-        else if (arg == "--help" || arg == "-h" || (!arg.empty() && arg.front() == '-'))
-            usage(argv[0]);
-        else
-            filename = arg;
+    const std::string usage = "[--path <dir>] [--proof <proof-file>] [--profile] <r1cs-file>";
+    const nkr::cli::Args args = nkr::cli::parse(argc, argv, nkr::cli::kPath | nkr::cli::kProfile, usage,
+                                                [&](const std::string &flag, const char *value) -> int {
+                                                    if (flag == "--proof" && value) {
+                                                        proof_input = value;
+                                                        return 2;
+                                                    }
+                                                    return 0;
+                                                });
+    if (args.positional.empty()) {
+        std::cerr << "Usage: " << argv[0] << ' ' << usage << '\n';
+        return 2;
     }
-    if (filename.empty()) usage(argv[0]);
+    const std::filesystem::path directory = args.directory;
+    const std::string filename = args.positional.back();
 
     try {
         Scope total("groth16.verify.total");

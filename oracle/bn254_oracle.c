@@ -30,6 +30,7 @@
  */
 #include "bn254_oracle.h"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -488,15 +489,14 @@ DEFINE_GROUP(g1, fe, fq)
 DEFINE_GROUP(g2, fe2, fq2)
 
 /* ------------------------------------------------------------ initialisation */
-static int g_init = 0;
-static void or_init(void)
+static pthread_once_t g_init_once = PTHREAD_ONCE_INIT;
+static void or_init_once(void)
 {
-    if (g_init) return;
     fe one;
     fq_one(&one);
     fq_neg(&FQ_NONRES, &one);
-    g_init = 1;
 }
+static void or_init(void) { pthread_once(&g_init_once, or_init_once); }
 
 /* ------------------------------------------------------------------ mt19937_64 */
 typedef struct {
@@ -982,4 +982,209 @@ void or_blind(const or_pk *pk, const void *raw, const void *r, const void *s, vo
     memcpy(o, &a, 96);
     memcpy(o + 96, &b, 192);
     memcpy(o + 288, &cc, 96);
+}
+
+/* ------------------------------------------------------------ all host cores */
+/* The reference's parallel MSM (operators/include/nkr/operators/
+ * variable_base_msm.hpp:17-49): workers = min(n, threads), chunk =
+ * ceil(n / workers); each worker runs single-threaded BDLO12 on its slice and
+ * the partial sums are added in worker order.  Here the chunks of several
+ * MSMs share one pool of `threads` POSIX threads taking tasks in order. */
+typedef struct {
+    int g2;           /* 0: G1 (96 B points), 1: G2 (192 B) */
+    const void *bases;
+    const fe *scalars;
+    size_t n;
+    void *out;        /* final sum */
+    size_t workers, chunk;
+    void *partial;    /* workers points */
+} msm_job;
+
+typedef struct {
+    msm_job *jobs;
+    size_t njobs, ntasks;
+    size_t next;
+    pthread_mutex_t lock;
+} msm_pool;
+
+static void msm_job_init(msm_job *j, int g2, const void *bases, const fe *scalars, size_t n, void *out,
+                         size_t threads)
+{
+    j->g2 = g2;
+    j->bases = bases;
+    j->scalars = scalars;
+    j->n = n;
+    j->out = out;
+    j->workers = n == 0 ? 0 : (threads < n ? (threads ? threads : 1) : n);
+    j->chunk = j->workers ? (n + j->workers - 1) / j->workers : 0;
+    j->partial = calloc(j->workers ? j->workers : 1, g2 ? sizeof(g2_t) : sizeof(g1_t));
+}
+
+static void *msm_pool_worker(void *arg)
+{
+    msm_pool *p = (msm_pool *)arg;
+    for (;;) {
+        pthread_mutex_lock(&p->lock);
+        const size_t t = p->next++;
+        pthread_mutex_unlock(&p->lock);
+        if (t >= p->ntasks) return NULL;
+        size_t k = t, j = 0;
+        while (k >= p->jobs[j].workers) k -= p->jobs[j++].workers;
+        msm_job *jb = &p->jobs[j];
+        const size_t begin = k * jb->chunk;
+        const size_t end = begin + jb->chunk < jb->n ? begin + jb->chunk : jb->n;
+        if (jb->g2) {
+            g2_t *part = (g2_t *)jb->partial + k;
+            if (begin >= end)
+                g2_zero(part);
+            else
+                g2_msm_bdlo12(part, (const g2_t *)jb->bases + begin, jb->scalars + begin, end - begin);
+        } else {
+            g1_t *part = (g1_t *)jb->partial + k;
+            if (begin >= end)
+                g1_zero(part);
+            else
+                g1_msm_bdlo12(part, (const g1_t *)jb->bases + begin, jb->scalars + begin, end - begin);
+        }
+    }
+}
+
+static void msm_run_jobs(msm_job *jobs, size_t njobs, size_t threads)
+{
+    msm_pool p;
+    p.jobs = jobs;
+    p.njobs = njobs;
+    p.ntasks = 0;
+    p.next = 0;
+    for (size_t j = 0; j < njobs; ++j) p.ntasks += jobs[j].workers;
+    pthread_mutex_init(&p.lock, NULL);
+    size_t nt = threads ? threads : 1;
+    if (nt > p.ntasks) nt = p.ntasks ? p.ntasks : 1;
+    pthread_t *th = (pthread_t *)malloc(nt * sizeof(pthread_t));
+    size_t started = 0;
+    for (; started + 1 < nt; ++started)
+        if (pthread_create(&th[started], NULL, msm_pool_worker, &p) != 0) break;
+    msm_pool_worker(&p); /* the calling thread works too */
+    for (size_t i = 0; i < started; ++i) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&p.lock);
+    for (size_t j = 0; j < njobs; ++j) {
+        msm_job *jb = &jobs[j];
+        if (jb->g2) {
+            g2_t acc;
+            g2_zero(&acc);
+            for (size_t k = 0; k < jb->workers; ++k) g2_add(&acc, &acc, (const g2_t *)jb->partial + k);
+            memcpy(jb->out, &acc, sizeof(acc));
+        } else {
+            g1_t acc;
+            g1_zero(&acc);
+            for (size_t k = 0; k < jb->workers; ++k) g1_add(&acc, &acc, (const g1_t *)jb->partial + k);
+            memcpy(jb->out, &acc, sizeof(acc));
+        }
+        free(jb->partial);
+        jb->partial = NULL;
+    }
+}
+
+int or_msm_g1_mt(const void *bases96, const void *scalars, size_t n, void *out96, size_t threads)
+{
+    or_init();
+    msm_job j;
+    msm_job_init(&j, 0, bases96, (const fe *)scalars, n, out96, threads);
+    msm_run_jobs(&j, 1, threads);
+    return 0;
+}
+
+int or_msm_g2_mt(const void *bases192, const void *scalars, size_t n, void *out192, size_t threads)
+{
+    or_init();
+    msm_job j;
+    msm_job_init(&j, 1, bases192, (const fe *)scalars, n, out192, threads);
+    msm_run_jobs(&j, 1, threads);
+    return 0;
+}
+
+typedef struct {
+    const or_csr *m;
+    const void *z;
+    fe *out;
+    int transform; /* also iFFT then cosetFFT the product in place */
+} poly_task;
+
+static void *poly_worker(void *arg)
+{
+    poly_task *t = (poly_task *)arg;
+    or_spmv(t->m, t->z, t->out);
+    return NULL;
+}
+
+static void *coset_worker(void *arg)
+{
+    poly_task *t = (poly_task *)arg;
+    const size_t n = t->m->rows;
+    domain_ifft(t->out, n);
+    or_fft(t->out, n, 2);
+    return NULL;
+}
+
+/* or_prove with the work spread over `threads` host threads: the three
+ * SpMVs and the three iFFT + cosetFFT pairs one thread per matrix, the
+ * pointwise quotient and icosetFFT on the calling thread, then the five
+ * MSMs chunked as variable_base_msm does.  Same results as or_prove. */
+int or_prove_mt(const or_circuit *c, const or_pk *pk, const void *witness, void *raw_out, void *az_out,
+                void *bz_out, void *cz_out, void *h_out, size_t threads)
+{
+    or_init();
+    const size_t n = c->a.rows, m = c->a.cols;
+    if (n < 2 || (n & (n - 1)) != 0 || or_log2(n) > FR_S) return -1;
+    if (threads == 0) threads = 1;
+    fe *poly[3];
+    for (int k = 0; k < 3; ++k) poly[k] = (fe *)malloc(n * sizeof(fe));
+    const or_csr *mats[3] = {&c->a, &c->b, &c->c};
+    poly_task tasks[3];
+    pthread_t th[3];
+    int started[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        tasks[k].m = mats[k];
+        tasks[k].z = witness;
+        tasks[k].out = poly[k];
+        started[k] = threads > 1 && pthread_create(&th[k], NULL, poly_worker, &tasks[k]) == 0;
+        if (!started[k]) poly_worker(&tasks[k]);
+    }
+    for (int k = 0; k < 3; ++k)
+        if (started[k]) pthread_join(th[k], NULL);
+    void *outs[3] = {az_out, bz_out, cz_out};
+    for (int k = 0; k < 3; ++k)
+        if (outs[k]) memcpy(outs[k], poly[k], n * sizeof(fe));
+    for (int k = 0; k < 3; ++k) {
+        started[k] = threads > 1 && pthread_create(&th[k], NULL, coset_worker, &tasks[k]) == 0;
+        if (!started[k]) coset_worker(&tasks[k]);
+    }
+    for (int k = 0; k < 3; ++k)
+        if (started[k]) pthread_join(th[k], NULL);
+    fe *a = poly[0], *b = poly[1], *cc = poly[2];
+    for (size_t i = 0; i < n; ++i) {
+        fr_mul(&a[i], &a[i], &b[i]);
+        fr_sub(&a[i], &a[i], &cc[i]);
+    }
+    fe g, gn, one, zi;
+    coset_gen(&g);
+    fr_pow_u64(&gn, &g, (uint64_t)n);
+    fr_one(&one);
+    fr_sub(&gn, &gn, &one);
+    fr_inv(&zi, &gn);
+    for (size_t i = 0; i < n; ++i) fr_mul(&a[i], &a[i], &zi);
+    or_fft(a, n, 3);
+    if (h_out) memcpy(h_out, a, n * sizeof(fe));
+    unsigned char *o = (unsigned char *)raw_out;
+    const fe *w = (const fe *)witness;
+    msm_job jobs[5];
+    msm_job_init(&jobs[0], 1, pk->b_g2_query, w, m, o + 384, threads);
+    msm_job_init(&jobs[1], 0, pk->a_query, w, m, o + 0, threads);
+    msm_job_init(&jobs[2], 0, pk->b_g1_query, w, m, o + 96, threads);
+    msm_job_init(&jobs[3], 0, pk->l_query, w, m, o + 192, threads);
+    msm_job_init(&jobs[4], 0, pk->h_query, a, n, o + 288, threads);
+    msm_run_jobs(jobs, 5, threads);
+    for (int k = 0; k < 3; ++k) free(poly[k]);
+    return 0;
 }

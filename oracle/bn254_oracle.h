@@ -95,6 +95,15 @@ typedef struct {
  * Optional az/bz/cz/h (n Fr each) receive the intermediates when non-NULL. */
 int or_prove(const or_circuit *c, const or_pk *pk, const void *witness, void *raw_out,
              void *az, void *bz, void *cz, void *h);
+/* The same on `threads` host threads (POSIX threads; 0 or 1 = one): the
+ * MSMs split into min(n, threads) contiguous chunks, each summed by
+ * BDLO12 and the partial sums added, as the reference's parallel
+ * variable_base_msm (operators/include/nkr/operators/variable_base_msm.hpp:
+ * 17-49).  Results equal the single-threaded ones (affine). */
+int or_msm_g1_mt(const void *bases96, const void *scalars, size_t n, void *out96, size_t threads);
+int or_msm_g2_mt(const void *bases192, const void *scalars, size_t n, void *out192, size_t threads);
+int or_prove_mt(const or_circuit *c, const or_pk *pk, const void *witness, void *raw_out,
+                void *az, void *bz, void *cz, void *h, size_t threads);
 /* blind_proof with caller-given r, s.  proof_out: A (G1), B (G2), C (G1) = 384 B. */
 void or_blind(const or_pk *pk, const void *raw, const void *r, const void *s, void *proof_out);
 

@@ -84,6 +84,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.or_prove.argtypes = [ctypes.POINTER(Circuit), ctypes.POINTER(Pk), vp, vp, vp, vp, vp, vp]
     L.or_prove.restype = ctypes.c_int
     L.or_blind.argtypes = [ctypes.POINTER(Pk), vp, vp, vp, vp]
+    L.or_msm_g1_mt.argtypes = [vp, vp, sz, vp, sz]
+    L.or_msm_g2_mt.argtypes = [vp, vp, sz, vp, sz]
+    L.or_prove_mt.argtypes = [ctypes.POINTER(Circuit), ctypes.POINTER(Pk), vp, vp, vp, vp, vp, vp, sz]
+    L.or_prove_mt.restype = ctypes.c_int
 
 
 def _p(a: np.ndarray) -> int:
@@ -285,6 +289,33 @@ def msm_g2(bases: np.ndarray, scalars: np.ndarray) -> np.ndarray:
     return out
 
 
+def host_threads() -> int:
+    """Host threads for the parallel oracle: the CPUs this process may run on,
+    at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def msm_g1_mt(bases: np.ndarray, scalars: np.ndarray, threads: int | None = None) -> np.ndarray:
+    """variable_base_msm: chunked BDLO12 over `threads` host threads."""
+    bases = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 12)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros(12, dtype=np.uint64)
+    lib().or_msm_g1_mt(_p(bases), _p(scalars), bases.shape[0], _p(out), threads or host_threads())
+    return out
+
+
+def msm_g2_mt(bases: np.ndarray, scalars: np.ndarray, threads: int | None = None) -> np.ndarray:
+    bases = np.ascontiguousarray(bases, dtype=np.uint64).reshape(-1, 24)
+    scalars = np.ascontiguousarray(scalars, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros(24, dtype=np.uint64)
+    lib().or_msm_g2_mt(_p(bases), _p(scalars), bases.shape[0], _p(out), threads or host_threads())
+    return out
+
+
 def _circuit(circ) -> Circuit:
     """circ: object with public_inputs, rows, cols, and a/b/c as (row_ptr, col_idx, values)."""
     mats = [make_csr(circ.rows, circ.cols, *m) for m in (circ.a, circ.b, circ.c)]
@@ -313,6 +344,20 @@ def prove(circ, pk, witness: np.ndarray, intermediates: bool = False):
     ptrs = [(_p(a) if a is not None else None) for a in inter]
     if lib().or_prove(ctypes.byref(c), ctypes.byref(k), _p(w), _p(raw), *ptrs) != 0:
         raise ValueError("or_prove failed")
+    return (raw, tuple(inter)) if intermediates else raw
+
+
+def prove_mt(circ, pk, witness: np.ndarray, intermediates: bool = False, threads: int | None = None):
+    """or_prove on several host threads (chunked MSMs); same results."""
+    c = _circuit(circ)
+    k = _pk(pk)
+    w = np.ascontiguousarray(witness, dtype=np.uint64).reshape(-1, 4)
+    raw = np.zeros(72, dtype=np.uint64)
+    n = circ.rows
+    inter = [np.zeros((n, 4), dtype=np.uint64) for _ in range(4)] if intermediates else [None] * 4
+    ptrs = [(_p(a) if a is not None else None) for a in inter]
+    if lib().or_prove_mt(ctypes.byref(c), ctypes.byref(k), _p(w), _p(raw), *ptrs, threads or host_threads()) != 0:
+        raise ValueError("or_prove_mt failed")
     return (raw, tuple(inter)) if intermediates else raw
 
 

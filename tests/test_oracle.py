@@ -192,3 +192,65 @@ def test_prove_and_blind_end_to_end():
     aff = np.concatenate([O.g1_to_affine(proof[:12])[0], O.g2_to_affine(proof[12:36])[0],
                           O.g1_to_affine(proof[36:])[0]])
     assert np.array_equal(aff, e.proof)
+
+
+# ------------------------------------------------------ parallel (all-core) oracle
+def test_msm_mt_fixtures_and_closed_form():
+    """The chunked MSM (variable_base_msm) gives the golden fixtures' points
+    for every thread count (chunks of one term, ragged last chunks, more
+    threads than terms), and the closed form at 2^11."""
+    for case in G["msm"]:
+        bases = goldens.points(case["bases"])
+        sc = goldens.fe(case["scalars"])
+        want = goldens.point(case["result"])
+        for threads in (1, 2, 3, 7, 64):
+            if case["group"] == "g1":
+                got = O.g1_to_affine(O.msm_g1_mt(bases, sc, threads))[0]
+            else:
+                got = O.g2_to_affine(O.msm_g2_mt(bases, sc, threads))[0]
+            assert np.array_equal(got, want), (case["group"], case["n"], case["flavour"], threads)
+    n = 1 << 11
+    k = O.fr_random(31, n)
+    s = O.fr_random(32, n)
+    s[::3] = 0
+    g1 = O.g1_generator()
+    bases = O.g1_mul(g1, k)
+    tot = sum(a * b for a, b in zip(O.fr_ints(k), O.fr_ints(s))) % O.R
+    expect = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
+    assert np.array_equal(O.g1_to_affine(O.msm_g1_mt(bases, s, 5))[0], expect)
+    empty = O.msm_g1_mt(np.zeros((0, 12), dtype=np.uint64), np.zeros((0, 4), dtype=np.uint64), 4)
+    assert O.words_to_int(empty[8:12]) == 0
+
+
+def test_prove_mt_equals_prove():
+    """or_prove_mt gives or_prove's intermediates and (affine) raw proof: the
+    golden matmul 2x2 proof, and a 2^10 Fibonacci chain with a random key."""
+    import circuits
+    e = goldens.E2E()
+    for threads in (1, 4):
+        raw, inter = O.prove_mt(e, e, e.witness, intermediates=True, threads=threads)
+        for got, key in zip(inter, ("az", "bz", "cz", "h")):
+            assert np.array_equal(got, goldens.fe(e.d[key]))
+        aff = np.concatenate([O.g1_to_affine(raw[:48].reshape(4, 12)).reshape(-1), O.g2_to_affine(raw[48:])[0]])
+        assert np.array_equal(aff, e.raw)
+    circ = circuits.fibonacci(1000, 0x4E4B52)
+    g1, g2 = O.g1_generator(), O.g2_generator()
+
+    class Pk:
+        pass
+    pk = Pk()
+    pk.a_query = O.g1_mul(g1, O.fr_random(1, circ.cols))
+    pk.b_g1_query = O.g1_mul(g1, O.fr_random(2, circ.cols))
+    pk.b_g2_query = O.g2_mul(g2, O.fr_random(3, circ.cols))
+    pk.l_query = O.g1_mul(g1, O.fr_random(4, circ.cols))
+    pk.h_query = O.g1_mul(g1, O.fr_random(5, circ.rows))
+    for k in ("alpha_g1", "beta_g1", "delta_g1"):
+        setattr(pk, k, g1)
+    for k in ("beta_g2", "delta_g2"):
+        setattr(pk, k, g2)
+    want_raw, want_inter = O.prove(circ, pk, circ.witness, intermediates=True)
+    got_raw, got_inter = O.prove_mt(circ, pk, circ.witness, intermediates=True, threads=6)
+    for a, b in zip(got_inter, want_inter):
+        assert np.array_equal(a, b)
+    aff = lambda r: np.concatenate([O.g1_to_affine(r[:48].reshape(4, 12)).reshape(-1), O.g2_to_affine(r[48:])[0]])
+    assert np.array_equal(aff(got_raw), aff(want_raw))
