@@ -387,24 +387,41 @@ def poseidon(count: int, seed: int, density: str = "full", folded: bool = False,
     return b.finish()
 
 
-def batch_mix(seed: int = 0x4E4B52, per_shape: int = 4):
+def batch_mix(seed: int = 0x4E4B52, per_shape: int = 4, use_cpp: bool | None = None):
     """The heterogeneous batch: 16 circuit shapes with domains 2^9 .. 2^18 --
     matmul of size 8 .. 48 (full and sparse), Fibonacci chains of 2^10 .. 2^16
     steps, Poseidon batches of 1 .. 512 hashes (one folded) -- each proved for
     `per_shape` different witnesses (the circuit structure does not depend on
-    the inputs, so one proving key serves all of them).
+    the inputs, so one proving key serves all of them).  use_cpp: generate
+    with the C++ generator (the same arrays, much faster); None uses it when
+    it is built.
     Returns [(name, circuit, [witness, ...])]."""
-    shapes = [("matmul8", lambda s: matmul(8, s)), ("matmul16", lambda s: matmul(16, s)),
-              ("matmul24_sparse", lambda s: matmul(24, s, density="sparse", wide=True)),
-              ("matmul32", lambda s: matmul(32, s)),
-              ("matmul40_sparse", lambda s: matmul(40, s, density="sparse", wide=True)),
-              ("matmul48", lambda s: matmul(48, s)),
-              ("fibonacci1024", lambda s: fibonacci(1 << 10, s)), ("fibonacci4096", lambda s: fibonacci(1 << 12, s)),
-              ("fibonacci16384", lambda s: fibonacci(1 << 14, s)), ("fibonacci65536", lambda s: fibonacci(1 << 16, s)),
-              ("poseidon1", lambda s: poseidon(1, s)), ("poseidon8", lambda s: poseidon(8, s)),
-              ("poseidon64_hybrid", lambda s: poseidon(64, s, density="hybrid")),
-              ("poseidon128_folded", lambda s: poseidon(128, s, folded=True)),
-              ("poseidon256", lambda s: poseidon(256, s)), ("poseidon512", lambda s: poseidon(512, s))]
+    cpp = os.path.exists(GEN_BIN) if use_cpp is None else use_cpp
+
+    def mm(s, sd, **kw):
+        if cpp:
+            return generate("matmul", s, sd, density=kw.get("density", "full"), wide=kw.get("wide", False))
+        return matmul(s, sd, **kw)
+
+    def fib(steps, sd):
+        return generate("fibonacci", steps, sd) if cpp else fibonacci(steps, sd)
+
+    def pos(count, sd, **kw):
+        if cpp:
+            return generate("poseidon", count, sd, density=kw.get("density", "full"), folded=kw.get("folded", False))
+        return poseidon(count, sd, **kw)
+
+    shapes = [("matmul8", lambda s: mm(8, s)), ("matmul16", lambda s: mm(16, s)),
+              ("matmul24_sparse", lambda s: mm(24, s, density="sparse", wide=True)),
+              ("matmul32", lambda s: mm(32, s)),
+              ("matmul40_sparse", lambda s: mm(40, s, density="sparse", wide=True)),
+              ("matmul48", lambda s: mm(48, s)),
+              ("fibonacci1024", lambda s: fib(1 << 10, s)), ("fibonacci4096", lambda s: fib(1 << 12, s)),
+              ("fibonacci16384", lambda s: fib(1 << 14, s)), ("fibonacci65536", lambda s: fib(1 << 16, s)),
+              ("poseidon1", lambda s: pos(1, s)), ("poseidon8", lambda s: pos(8, s)),
+              ("poseidon64_hybrid", lambda s: pos(64, s, density="hybrid")),
+              ("poseidon128_folded", lambda s: pos(128, s, folded=True)),
+              ("poseidon256", lambda s: pos(256, s)), ("poseidon512", lambda s: pos(512, s))]
     out = []
     for k, (name, make) in enumerate(shapes):
         circ = make(seed + 1000 * k)

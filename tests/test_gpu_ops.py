@@ -84,6 +84,33 @@ def test_ntt_round_trip_2_24():
     assert (ones == np.array(one, dtype=np.uint64)).all()
 
 
+def test_ntt_2_24_monomial_pins_omega():
+    """The transforms of x at 2^24 pin the root libfqfft uses: FFT(x)[i] =
+    omega^i with omega = root_of_unity^(2^(28-24)) (field_utils.tcc:38-51; the
+    2^28-th root from alt_bn128_init.cpp:60), cosetFFT(x)[i] = 5 omega^i, and
+    the inverse transforms of those vectors return x.  A transform on another
+    primitive root, or with permuted twiddles, fails the ratio check."""
+    n, log_n = 1 << 24, 24
+    root28 = 19103219067921713944291392827692070036145651957329286315305642004821462161904
+    assert pow(root28, 1 << 27, O.R) == O.R - 1  # order exactly 2^28
+    omega = pow(root28, 1 << (28 - log_n), O.R)
+    mont = lambda v: np.array(O.int_to_words(O.to_mont_int(v, O.R)), dtype=np.uint64)
+    x = np.zeros((n, 4), dtype=np.uint64)
+    x[1] = mont(1)
+    f = N.ntt(x, N.FFT)
+    assert np.array_equal(f[0], mont(1)) and np.array_equal(f[1], mont(omega))
+    # f[i + 1] = f[i] * omega for every i (and f wraps: f[n-1] * omega = 1)
+    assert np.array_equal(O.fr_mul(f, np.broadcast_to(mont(omega), f.shape).copy()), np.roll(f, -1, axis=0))
+    rng = np.random.default_rng(24)
+    for i in rng.integers(0, n, 16).tolist() + [n // 2, n - 1]:
+        assert np.array_equal(f[i], mont(pow(omega, i, O.R))), i
+    c = N.ntt(x, N.COSET_FFT)
+    assert np.array_equal(c, O.fr_mul(f, np.broadcast_to(mont(5), f.shape).copy()))
+    assert np.array_equal(c[3], mont(5 * pow(omega, 3, O.R) % O.R))
+    assert np.array_equal(N.ntt(f, N.IFFT), x)
+    assert np.array_equal(N.ntt(c, N.ICOSET_FFT), x)
+
+
 def test_quotient_golden_and_oracle():
     for case in G["quotient"]:
         h = N.quotient(goldens.fe(case["az"]), goldens.fe(case["bz"]), goldens.fe(case["cz"]))
@@ -250,44 +277,62 @@ def witness_like(s, seed):
     return s
 
 
+def dot_closed_form(k, s):
+    """sum_i k_i s_i mod r for Montgomery-form vectors: the products by the
+    oracle's Montgomery multiplication (Mont(k_i s_i)), summed exactly by
+    block_sums; the first 256 terms are re-derived with Python integers."""
+    prod = O.fr_mul(k, s)
+    head = sum(a * b for a, b in zip(O.fr_ints(k[:256]), O.fr_ints(s[:256]))) % O.R
+    assert block_sums(prod[:256], 256)[0] == head
+    return block_sums(prod, prod.shape[0])[0]
+
+
 @pytest.mark.parametrize("log_n", [20, 24])
 def test_msm_g1_operator_sizes_closed_form(log_n):
-    """G1 MSM at 2^20 and 2^24 terms (c = 16, 16 windows: 2^28 bucket entries
-    and 16 GiB of window-shifted bases at 2^24), uniform and witness-like
-    scalars.  Bases k_j G repeat over runs of 64 terms, so
-    sum_i s_i B_i = (sum_j k_j T_j) G with T_j the run sums of the scalars;
-    a run's repeated base also drives the doubling and cancelling cases of
-    the bucket additions."""
-    n, block = 1 << log_n, 64
+    """G1 MSM at 2^20 and 2^24 distinct bases k_i G (c = 16, 16 windows: 2^28
+    bucket entries and 16 GiB of window-shifted bases at 2^24), uniform and
+    witness-like scalars: sum_i s_i (k_i G) = (sum_i k_i s_i) G."""
+    n = 1 << log_n
     g1 = O.g1_generator()
-    k = rand_fr(1200 + log_n, n // block)
-    bases = np.repeat(N.fixed_base_mul(g1, k), block, axis=0)
-    m = N.Msm(bases)
-    del bases
-    kk = O.fr_ints(k)
+    k = rand_fr(1200 + log_n, n)
+    m = N.Msm(N.fixed_base_mul(g1, k))
     s = rand_fr(1300 + log_n, n)
     for variant in ("uniform", "witness"):
         if variant == "witness":
             s = witness_like(s, 1400 + log_n)
-        t = block_sums(s, block)
-        check_block_sums(s, t, block)
-        tot = sum(a * b for a, b in zip(kk, t)) % O.R
-        want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
+        want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([dot_closed_form(k, s)])))[0]
         assert np.array_equal(m.run(s), want), (log_n, variant)
     m.close()
 
 
-def test_msm_g2_2_18_closed_form():
-    """G2 MSM at 2^18 terms with uniform scalars, closed form as for G1."""
-    n, block = 1 << 18, 64
-    g2 = O.g2_generator()
-    k = rand_fr(1500, n // block)
-    bases = np.repeat(N.fixed_base_mul(g2, k, g2=True), block, axis=0)
-    s = rand_fr(1501, n)
+def test_msm_g1_repeated_bases_closed_form():
+    """2^20 terms whose bases repeat over runs of 64 (k_j G for run j): the
+    repeated base drives the doubling and cancelling cases of the bucket
+    additions; sum_i s_i B_i = (sum_j k_j T_j) G with T_j the run sums."""
+    n, block = 1 << 20, 64
+    g1 = O.g1_generator()
+    k = rand_fr(1220, n // block)
+    bases = np.repeat(N.fixed_base_mul(g1, k), block, axis=0)
+    m = N.Msm(bases)
+    del bases
+    kk = O.fr_ints(k)
+    s = witness_like(rand_fr(1320, n), 1420)
     t = block_sums(s, block)
     check_block_sums(s, t, block)
-    tot = sum(a * b for a, b in zip(O.fr_ints(k), t)) % O.R
-    want = O.g2_to_affine(O.g2_mul(g2, O.fr_array([tot])))[0]
+    tot = sum(a * b for a, b in zip(kk, t)) % O.R
+    want = O.g1_to_affine(O.g1_mul(g1, O.fr_array([tot])))[0]
+    assert np.array_equal(m.run(s), want)
+    m.close()
+
+
+def test_msm_g2_2_18_closed_form():
+    """G2 MSM at 2^18 distinct bases k_i G2 with uniform scalars."""
+    n = 1 << 18
+    g2 = O.g2_generator()
+    k = rand_fr(1500, n)
+    bases = N.fixed_base_mul(g2, k, g2=True)
+    s = rand_fr(1501, n)
+    want = O.g2_to_affine(O.g2_mul(g2, O.fr_array([dot_closed_form(k, s)])))[0]
     assert np.array_equal(N.msm(bases, s, g2=True), want)
 
 
