@@ -271,6 +271,17 @@ int nkr_g16_quotient(const void *az, const void *bz, const void *cz, uint64_t ro
 int nkr_fr_ntt(void *data, uint32_t log_n, int kind, int device);               /* host memory */
 int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream);    /* device memory */
 
+/* As nkr_fr_ntt_dev, allocating no device memory: for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The tables and the
+ * reordering buffer live in one fixed device array per GPU (64.2 MiB, held
+ * until the process ends), so log_n is limited to 1 .. 20; larger sizes fail
+ * with an error.  The tables are rebuilt on `stream` by every call.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller.
+ * Results equal nkr_fr_ntt_dev's. */
+int nkr_fr_ntt_dev_noalloc(void *data_dev, uint32_t log_n, int kind, void *stream);
+
 /* NTT workgroup shape for contexts and domains created afterwards:
  * 2^lds_log elements per workgroup (10 or 11), 2^radix_log elements per
  * thread per register round (2 or 3), and how transforms larger than one

@@ -28,6 +28,20 @@ std::mutex g_ntt_mutex;
 nkr::NttDomain *g_ntt_cache[kMaxDevices][29] = {};
 nkr::DeviceBuffer<nkr::fr_t> *g_ntt_scratch[kMaxDevices] = {};
 
+// Device scratch of the allocation-free transform (nkr_fr_ntt_dev_noalloc):
+// the table arena of a 2^20-point domain followed by a 2^20-element buffer
+// for the final reordering -- 2^21 + 6144 elements of 32 B (64.2 MiB), one
+// copy per device for the life of the process.  Host code never names
+// `scratch` in a kernel argument or copy: in host code the name is a stand-in,
+// not a device address.  Each device's address comes from
+// hipGetSymbolAddress (noalloc_scratch) with that device current.
+constexpr uint32_t kNoAllocMaxLog = 20;
+constexpr size_t kNoAllocArenaElems = nkr::NttDomain::standalone_arena_elems(kNoAllocMaxLog);
+constexpr size_t kNoAllocElems = kNoAllocArenaElems + (size_t(1) << kNoAllocMaxLog);
+static __device__ __align__(256) uint4 scratch[kNoAllocElems * sizeof(nkr::fr_t) / sizeof(uint4)];
+std::mutex g_noalloc_mutex;
+nkr::fr_t *g_noalloc_ptr[kMaxDevices] = {};
+
 template <class Fn>
 int guarded(Fn &&fn)
 {
@@ -511,6 +525,41 @@ int nkr_fr_ntt_dev(void *data_dev, uint32_t log_n, int kind, void *stream)
         if (!scratch) scratch = new DeviceBuffer<fr_t>();
         scratch->ensure(size_t(1) << log_n);
         dom->standalone(static_cast<fr_t *>(data_dev), scratch->get(), kind, s);
+    });
+}
+
+// The current device's copy of `scratch`, looked up once per device.
+static fr_t *noalloc_scratch(int dev)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        NKR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NKR_REQUIRE(p, "no device address for the transform scratch");
+        g_noalloc_ptr[dev] = static_cast<fr_t *>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
+
+int nkr_fr_ntt_dev_noalloc(void *data_dev, uint32_t log_n, int kind, void *stream)
+{
+    return guarded([&] {
+        NKR_REQUIRE(data_dev, "null data");
+        NKR_REQUIRE(log_n >= 1 && log_n <= kNoAllocMaxLog,
+                    "nkr_fr_ntt_dev_noalloc transforms 2^1 .. 2^20 elements (the size of its fixed device "
+                    "scratch); use nkr_fr_ntt_dev for larger transforms");
+        NKR_REQUIRE(kind >= 0 && kind <= 3, "unknown transform kind (0 FFT, 1 iFFT, 2 cosetFFT, 3 icosetFFT)");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        int dev = 0;
+        NKR_HIP(hipGetDevice(&dev));
+        NKR_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index out of range");
+        fr_t *arena = noalloc_scratch(dev);
+        static_assert(NttDomain::standalone_arena_elems(kNoAllocMaxLog) == kNoAllocArenaElems, "arena size");
+        NKR_REQUIRE(NttDomain::standalone_arena_elems(log_n) <= kNoAllocArenaElems, "transform scratch too small");
+        // Tables are rebuilt on the stream by every call (they are exact, so
+        // the results equal nkr_fr_ntt_dev's); nothing needs clearing.
+        const NttDomain dom(log_n, kind, arena, s);
+        dom.standalone(static_cast<fr_t *>(data_dev), arena + kNoAllocArenaElems, kind, s);
     });
 }
 

@@ -32,6 +32,19 @@ struct NttShape {
 class NttDomain {
 public:
     NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape = nullptr);
+    // A domain for standalone transforms of one `kind` (0 FFT, 1 iFFT,
+    // 2 cosetFFT, 3 icosetFFT) whose tables are built in `arena`, caller-owned
+    // device memory of standalone_arena_elems(log_n) elements, instead of
+    // allocated: the construction and its transforms allocate no device
+    // memory.  Only that kind's direction and coset table exist.
+    NttDomain(unsigned log_n, int kind, fr_t *arena, hipStream_t stream);
+    // Elements an arena needs for any kind at 2^log_n points: twiddles and
+    // their two-level power tables (2 x (lo + hi)), sub-transform roots
+    // (<= 2^11) and one coset table (2^log_n).
+    static constexpr size_t standalone_arena_elems(unsigned log_n)
+    {
+        return 2 * ((size_t(1) << ((log_n + 1) / 2)) + (size_t(1) << (log_n / 2))) + 2048 + (size_t(1) << log_n);
+    }
 
     unsigned log_n() const { return log_n_; }
     size_t size() const { return size_t(1) << log_n_; }
@@ -65,8 +78,8 @@ public:
     void quotient(const fr_t *a, const fr_t *b, const fr_t *c, fr_t *h, hipStream_t stream) const;
 
     // Tables the prove chain fuses into its passes.
-    const fr_t *coset_rev_scaled() const { return coset_rev_.get(); }       // g^rev(j) / N
-    const fr_t *icoset_rev_scaled() const { return icoset_rev_.get(); }     // g^-rev(j) / N
+    const fr_t *coset_rev_scaled() const { return coset_rev_p_; }      // g^rev(j) / N
+    const fr_t *icoset_rev_scaled() const { return icoset_rev_p_; }    // g^-rev(j) / N
 
     fr_t omega() const { return omega_[0]; }
 
@@ -76,6 +89,8 @@ private:
     };
     void run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bool inverse, const fr_t *pre,
              const fr_t *post, const fr_t *post_const, hipStream_t stream) const;
+    // Pass split, constants and table sizes (both constructors).
+    void plan(unsigned log_n, const NttShape *shape);
 
     unsigned log_n_;
     unsigned lds_log_ = 11, radix_log_ = 3;
@@ -85,8 +100,13 @@ private:
     fr_t omega_[2];        // omega, omega^-1
     fr_t n_inv_;
     fr_t zinv_;            // (g^N - 1)^-1
+    uint32_t lo_count_ = 0, hi_count_ = 0, root_count_ = 0;
+    int arena_kind_ = -1;  // the kind an arena domain serves; -1 when the tables are owned
     DeviceBuffer<fr_t> tw_lo_[2], tw_hi_[2], roots_[2];
-    DeviceBuffer<fr_t> coset_nat_, coset_rev_, icoset_rev_, consts_;
+    DeviceBuffer<fr_t> coset_nat_, coset_rev_, icoset_rev_;
+    // The tables in use: the buffers above, or places in an arena (null when absent).
+    const fr_t *tw_lo_p_[2] = {}, *tw_hi_p_[2] = {}, *roots_p_[2] = {};
+    const fr_t *coset_nat_p_ = nullptr, *coset_rev_p_ = nullptr, *icoset_rev_p_ = nullptr;
 };
 
 // Workgroup shape for domains built afterwards: 2^lds_log LDS elements per

@@ -341,6 +341,20 @@ __global__ void k_pow_rev_table(fr_t *out, fr_t base, fr_t scale, uint32_t log_n
     out[j] = acc;
 }
 
+// out[j] = scale * base^e, e = j (or bitrev(j) when rev), from the two-level
+// tables lo[x] = base^x (x < 2^m) and hi[y] = base^(2^m y): one product per
+// element (two with a scale) instead of an exponentiation.
+__global__ void k_pow_two_level(fr_t *out, const fr_t *__restrict__ hi, const fr_t *__restrict__ lo, fr_t scale,
+                                uint32_t m, uint32_t log_n, uint32_t rev, uint32_t has_scale)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (1u << log_n)) return;
+    const uint32_t e = rev ? brev_bits(j, log_n) : j;
+    fr_t v = fp_mul(hi[e >> m], lo[e & ((1u << m) - 1u)]);
+    if (has_scale) v = fp_mul(v, scale);
+    out[j] = v;
+}
+
 __global__ void k_bitrev(const fr_t *__restrict__ in, fr_t *__restrict__ out, uint32_t log_n)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -410,7 +424,7 @@ fr_t fr_root_of_unity(unsigned log_n)
     return w;
 }
 
-NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) : log_n_(log_n)
+void NttDomain::plan(unsigned log_n, const NttShape *shape)
 {
     NKR_REQUIRE(log_n >= 1 && log_n <= kTwoAdicity, "NTT size must be 2^1 .. 2^28");
     const size_t n = size_t(1) << log_n;
@@ -418,7 +432,6 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) 
     omega_[1] = fp_inv(omega_[0]);
     n_inv_ = fp_inv(fr_from_u64(n));
     const fr_t g = fr_from_u64(5);  // multiplicative generator (alt_bn128_init.cpp:55)
-    const fr_t g_inv = fp_inv(g);
     zinv_ = fp_inv(fp_sub(fr_pow_u64(g, n), fr_t::one()));
 
     // pass plan: sub-transforms of at most lds_log bits
@@ -462,21 +475,39 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) 
 
     m_lo_ = (log_n + 1) / 2;
     log_roots_ = std::min(log_n, 12u);
-    const uint32_t lo_count = 1u << m_lo_;
-    const uint32_t hi_count = 1u << (log_n - m_lo_);
-    const uint32_t root_count = std::max(1u, 1u << (log_roots_ - 1));
+    lo_count_ = 1u << m_lo_;
+    hi_count_ = 1u << (log_n - m_lo_);
+    root_count_ = std::max(1u, 1u << (log_roots_ - 1));
+}
+
+// The inter-pass twiddle tables and sub-transform roots of one direction.
+static void build_twiddles(fr_t *lo, fr_t *hi, fr_t *roots, const fr_t &omega, uint32_t lo_count, uint32_t hi_count,
+                           uint32_t root_count, unsigned log_n, unsigned log_roots, hipStream_t stream)
+{
+    k_pow_table<<<div_up(lo_count, 256), 256, 0, stream>>>(lo, omega, lo_count);
+    NKR_LAUNCH_CHECK();
+    k_pow_table<<<div_up(hi_count, 256), 256, 0, stream>>>(hi, fr_pow_u64(omega, lo_count), hi_count);
+    NKR_LAUNCH_CHECK();
+    const fr_t root = fr_pow_u64(omega, uint64_t(1) << (log_n - log_roots));
+    k_pow_table<<<div_up(root_count, 256), 256, 0, stream>>>(roots, root, root_count);
+    NKR_LAUNCH_CHECK();
+}
+
+NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) : log_n_(log_n)
+{
+    plan(log_n, shape);
+    const size_t n = size_t(1) << log_n;
+    const fr_t g = fr_from_u64(5);
+    const fr_t g_inv = fp_inv(g);
     for (int d = 0; d < 2; ++d) {
-        tw_lo_[d].allocate(lo_count);
-        tw_hi_[d].allocate(hi_count);
-        roots_[d].allocate(root_count);
-        k_pow_table<<<div_up(lo_count, 256), 256, 0, stream>>>(tw_lo_[d].get(), omega_[d], lo_count);
-        NKR_LAUNCH_CHECK();
-        k_pow_table<<<div_up(hi_count, 256), 256, 0, stream>>>(tw_hi_[d].get(), fr_pow_u64(omega_[d], lo_count),
-                                                              hi_count);
-        NKR_LAUNCH_CHECK();
-        const fr_t root = fr_pow_u64(omega_[d], uint64_t(1) << (log_n - log_roots_));
-        k_pow_table<<<div_up(root_count, 256), 256, 0, stream>>>(roots_[d].get(), root, root_count);
-        NKR_LAUNCH_CHECK();
+        tw_lo_[d].allocate(lo_count_);
+        tw_hi_[d].allocate(hi_count_);
+        roots_[d].allocate(root_count_);
+        build_twiddles(tw_lo_[d].get(), tw_hi_[d].get(), roots_[d].get(), omega_[d], lo_count_, hi_count_,
+                       root_count_, log_n, log_roots_, stream);
+        tw_lo_p_[d] = tw_lo_[d].get();
+        tw_hi_p_[d] = tw_hi_[d].get();
+        roots_p_[d] = roots_[d].get();
     }
     coset_nat_.allocate(n);
     coset_rev_.allocate(n);
@@ -487,6 +518,41 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) 
     NKR_LAUNCH_CHECK();
     k_pow_rev_table<<<div_up(n, 256), 256, 0, stream>>>(icoset_rev_.get(), g_inv, n_inv_, log_n);
     NKR_LAUNCH_CHECK();
+    coset_nat_p_ = coset_nat_.get();
+    coset_rev_p_ = coset_rev_.get();
+    icoset_rev_p_ = icoset_rev_.get();
+}
+
+NttDomain::NttDomain(unsigned log_n, int kind, fr_t *arena, hipStream_t stream) : log_n_(log_n)
+{
+    NKR_REQUIRE(kind >= 0 && kind <= 3, "unknown transform kind (0 FFT, 1 iFFT, 2 cosetFFT, 3 icosetFFT)");
+    NKR_REQUIRE(arena, "null table arena");
+    plan(log_n, nullptr);
+    arena_kind_ = kind;
+    const int d = (kind == 1 || kind == 3) ? 1 : 0;
+    fr_t *p = arena;
+    fr_t *lo = p, *hi = lo + lo_count_, *roots = hi + hi_count_;
+    p = roots + root_count_;
+    build_twiddles(lo, hi, roots, omega_[d], lo_count_, hi_count_, root_count_, log_n, log_roots_, stream);
+    tw_lo_p_[d] = lo;
+    tw_hi_p_[d] = hi;
+    roots_p_[d] = roots;
+    if (kind >= 2) {
+        // g^i (cosetFFT, natural order) or g^-rev(j) / N (icosetFFT, applied
+        // after the bit-reversed output), from two-level power tables of g^(+-1)
+        const size_t n = size_t(1) << log_n;
+        const fr_t base = kind == 2 ? fr_from_u64(5) : fp_inv(fr_from_u64(5));
+        fr_t *glo = p, *ghi = glo + lo_count_, *table = ghi + hi_count_;
+        NKR_REQUIRE(size_t(table + n - arena) <= standalone_arena_elems(log_n), "table arena layout overflow");
+        k_pow_table<<<div_up(lo_count_, 256), 256, 0, stream>>>(glo, base, lo_count_);
+        NKR_LAUNCH_CHECK();
+        k_pow_table<<<div_up(hi_count_, 256), 256, 0, stream>>>(ghi, fr_pow_u64(base, lo_count_), hi_count_);
+        NKR_LAUNCH_CHECK();
+        k_pow_two_level<<<div_up(n, 256), 256, 0, stream>>>(table, ghi, glo, n_inv_, m_lo_, log_n,
+                                                            kind == 3 ? 1u : 0u, kind == 3 ? 1u : 0u);
+        NKR_LAUNCH_CHECK();
+        (kind == 2 ? coset_nat_p_ : icoset_rev_p_) = table;
+    }
 }
 
 void NttDomain::run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bool inverse, const fr_t *pre,
@@ -494,6 +560,7 @@ void NttDomain::run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bo
 {
     NKR_REQUIRE(count >= 1 && count <= 65535, "NTT batch must hold 1 .. 65535 transforms");
     const int d = inverse ? 1 : 0;
+    NKR_REQUIRE(tw_lo_p_[d], "this transform direction has no tables in this domain");
     const int P = (int)passes_.size();
     for (int k = 0; k < P; ++k) {
         const int pi = dit ? (P - 1 - k) : k;
@@ -501,9 +568,9 @@ void NttDomain::run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bo
         PassArgs a{};
         a.data = data;
         a.batch = batch;
-        a.tw_lo = tw_lo_[d].get();
-        a.tw_hi = tw_hi_[d].get();
-        a.roots = roots_[d].get();
+        a.tw_lo = tw_lo_p_[d];
+        a.tw_hi = tw_hi_p_[d];
+        a.roots = roots_p_[d];
         a.pre = (k == 0) ? pre : nullptr;
         a.post = (k == P - 1) ? post : nullptr;
         a.has_post_const = (k == P - 1 && post_const) ? 1u : 0u;
@@ -575,11 +642,12 @@ void NttDomain::bitrev(const fr_t *in, fr_t *out, hipStream_t stream) const
 
 void NttDomain::standalone(fr_t *data, fr_t *scratch, int kind, hipStream_t stream) const
 {
+    NKR_REQUIRE(arena_kind_ < 0 || kind == arena_kind_, "this domain's tables serve another transform kind");
     switch (kind) {
     case 0: dif(data, false, nullptr, nullptr, nullptr, stream); break;
     case 1: dif(data, true, nullptr, nullptr, &n_inv_, stream); break;
-    case 2: dif(data, false, coset_nat_.get(), nullptr, nullptr, stream); break;
-    case 3: dif(data, true, nullptr, icoset_rev_.get(), nullptr, stream); break;
+    case 2: dif(data, false, coset_nat_p_, nullptr, nullptr, stream); break;
+    case 3: dif(data, true, nullptr, icoset_rev_p_, nullptr, stream); break;
     default: throw Error("unknown transform kind (0 FFT, 1 iFFT, 2 cosetFFT, 3 icosetFFT)");
     }
     bitrev(data, scratch, stream);

@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],
             "nkr_fr_ntt": [vp, u32, i32, i32],
             "nkr_fr_ntt_dev": [vp, u32, i32, vp],
+            "nkr_fr_ntt_dev_noalloc": [vp, u32, i32, vp],
             "nkr_spmv": [u64, u64, vp, vp, vp, u64, vp, vp, i32],
             "nkr_spmv_create": [u64, u64, vp, vp, vp, u64, i32, ctypes.POINTER(vp)],
             "nkr_spmv_run_dev": [vp, vp, vp, vp],
