@@ -182,15 +182,16 @@ int nkr_g16_server_destroy(nkr_g16_server *server);
  *   0 SpMV, 1 NTT passes (6 transforms: 3 iFFT, 3 cosetFFT; the H query is
  *   prepared in the coset-evaluation basis, so no icosetFFT runs), 2 quotient
  *   pointwise, 3 MSM digit sort (2 schedules: witness and quotient),
- *   4 MSM bucket accumulation (2 launches: one G1 batch of ar, bs1, zk and
- *   H; the G2 MSM bs2), 5 MSM folding, completion and bit planes, 6 plane
- *   download + host finish.  Stages on the two streams overlap, so the
+ *   4 G1 MSM bucket accumulation (the k_accumulate launch of the G1 batch:
+ *   ar, bs1, zk and H), 5 MSM folding, completion and bit planes (G1 and
+ *   G2), 6 plane download + host finish, 7 G2 MSM bucket accumulation (the
+ *   k_accumulate launch of bs2).  Stages on the two streams overlap, so the
  *   totals do not add up to the proof's latency.
  * counters[0] = proves timed, counters[1] = accumulation launches,
  * counters[2] = bucket entries (non-zero signed digits) of the witness
  * schedule, summed over proves (each is used by 3 G1 MSMs and the G2 MSM),
  * counters[3] = bucket entries of the H schedule (the fifth MSM). */
-#define NKR_G16_STAGES 7
+#define NKR_G16_STAGES 8
 int nkr_g16_profile(nkr_g16_ctx *ctx, int enable);
 int nkr_g16_profile_read(const nkr_g16_ctx *ctx, double *stage_ms, uint64_t *counters);
 
@@ -358,6 +359,12 @@ int nkr_device_synchronize(void);
 /* Plain streaming copy kernel (16-byte vectors): a known-traffic reference
  * for calibrating hardware byte counters.  bytes must be a multiple of 16. */
 int nkr_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, void *stream);
+/* Integer-VALU calibration on `device` (no counterpart in the reference, whose
+ * roofs are not measured): the issue rate of v_mad_u64_u32 over the whole
+ * device, the 32x32+64-bit multiply-add every Montgomery product is built
+ * from, and this library's Fr Montgomery products per second.  Takes a few
+ * milliseconds. */
+int nkr_valu_probe(int device, double *mad_per_s, double *fr_mul_per_s);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 #include "msm.h"
 #include "ntt.h"
 #include "pairing.h"
+#include "probe.h"
 #include "prover.h"
 #include "serve.h"
 #include "setup.h"
@@ -892,6 +893,11 @@ int nkr_dev_copy(void *dst, const void *src, uint64_t bytes, void *stream)
                                                                         static_cast<uint4 *>(dst), n);
         NKR_LAUNCH_CHECK();
     });
+}
+
+int nkr_valu_probe(int device, double *mad_per_s, double *fr_mul_per_s)
+{
+    return guarded([&] { valu_probe(device, mad_per_s, fr_mul_per_s); });
 }
 
 }  // extern "C"

@@ -157,7 +157,7 @@ private:
 };
 
 // Stage indices used with StageTimer by the MSM code.
-enum MsmStage { kStageSort = 3, kStageAccumulate = 4, kStageFold = 5 };
+enum MsmStage { kStageSort = 3, kStageAccumulate = 4, kStageFold = 5, kStageAccumulateG2 = 7 };
 
 template <class F>
 struct MsmJob {

@@ -1,6 +1,8 @@
 // Pippenger MSM kernels for gfx950 (see msm.h for the design).
 #include "msm.h"
 
+#include "curve29.h"
+
 #include <algorithm>
 #include <array>
 #include <cstring>
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
         const Affine<F> *__restrict__ bases = jb.bases;
         const uint32_t *__restrict__ sorted = jb.sorted;
         Xyzz<F> acc = Xyzz<F>::infinity();
-        if (std::is_same<F, fq2_t>::value) {
+        if constexpr (std::is_same<F, fq2_t>::value) {
             // G2: no prefetch; the second base's 32 registers would push the
             // addition's working set past the register file of two waves.
 #pragma unroll 1
@@ -436,9 +438,13 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
                 }
             }
         } else if (beg < end) {
-            // Software pipeline: the gather of entry e + 1's base is issued
-            // before entry e's addition, so its latency hides behind the
-            // field arithmetic.
+            // G1: the accumulator stays in the reduced-radix form
+            // (curve29.h: one v_mad_u64_u32 per product term, no carries)
+            // for the whole piece; bases are converted as they arrive and
+            // the partial sum once at the end.  Software pipeline: the
+            // gather of entry e + 1's base is issued before entry e's
+            // addition, so its latency hides behind the field arithmetic.
+            Xyzz29<FqParams> acc29 = Xyzz29<FqParams>::infinity();
             uint32_t v = sorted[beg];
             Affine<F> P = bases[v & kEntryMask];
 #pragma unroll 1
@@ -451,11 +457,12 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
                 }
                 if (!affine_is_inf(P)) {
                     const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
-                    acc = xyzz_add_affine(acc, P.x, y);
+                    acc29 = xyzz29_add_affine(acc29, fp29_from_mont(P.x), fp29_from_mont(y));
                 }
                 v = vn;
                 P = Pn;
             }
+            acc = xyzz29_to(acc29);
         }
         jb.pool[p] = acc;
     }
@@ -1004,14 +1011,16 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
 
     k_regions_init<F><<<dim3(div_up(Bmax, 256), J), 256, 0, stream>>>(jd, class_count_.get());
     NKR_LAUNCH_CHECK();
-    if (timer) timer->begin(kStageAccumulate, stream);
     k_piece_scan<F><<<1, 1024, 0, stream>>>(jd, J, thread_base_dev_.get());
     NKR_LAUNCH_CHECK();
+    // the accumulation stage brackets exactly the k_accumulate launch
+    const int acc_stage = std::is_same<F, fq2_t>::value ? kStageAccumulateG2 : kStageAccumulate;
+    if (timer) timer->begin(acc_stage, stream);
     const uint64_t blocks = std::min<uint64_t>(div_up(thread_total_, 256), resident_blocks<k_accumulate<F>>());
     k_accumulate<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
     NKR_LAUNCH_CHECK();
     if (timer) {
-        timer->end(kStageAccumulate, stream);
+        timer->end(acc_stage, stream);
         timer->begin(kStageFold, stream);
     }
     // Fold levels the largest bucket of any job needs: read from the host

@@ -19,6 +19,7 @@
 #include "msm.h"
 #include "ntt.h"
 #include "spmv.h"
+#include "upload.h"
 
 namespace nkr {
 
@@ -44,7 +45,7 @@ public:
     void prove_blind(const void *witness, bool on_device, const fr_t &r, const fr_t &s, void *raw_out,
                      void *proof_out);
 
-    static constexpr int kStages = 7;
+    static constexpr int kStages = 8;
     void set_profiling(bool on);
     void profile_read(double *stage_ms, uint64_t *counters) const;
 
@@ -78,7 +79,7 @@ private:
     DeviceBuffer<Xyzz<fq2_t>> planes2_;
     PinnedStaging<Xyzz<fq_t>> planes1_host_;
     PinnedStaging<Xyzz<fq2_t>> planes2_host_;
-    void *pinned_witness_ = nullptr;
+    std::unique_ptr<WitnessUpload> upload_;  // host witnesses: packed, chunked upload
 
     DeviceBuffer<fr_t *> poly_table_;  // {pa, pb, pc} for the batched transforms
 

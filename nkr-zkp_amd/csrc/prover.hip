@@ -182,7 +182,7 @@ Groth16Prover::Groth16Prover(const nkr_g16_circuit &c, const nkr_g16_pk &pk, int
     NKR_HIP(hipMemcpyAsync(poly_table_.get(), table, sizeof(table), hipMemcpyHostToDevice, stream_));
     planes1_.allocate(3 * wplan_.c + hplan_.c);
     planes2_.allocate(wplan_.c);
-    NKR_HIP(hipHostMalloc(&pinned_witness_, (size_t)cols_ * sizeof(fr_t), hipHostMallocDefault));
+    upload_ = std::make_unique<WitnessUpload>(cols_);
     NKR_HIP(hipStreamSynchronize(stream_));
     helpers_ = std::make_unique<HostHelpers>(2);
     // The witness G1 MSMs read the schedule in pieces of at least 32 entries
@@ -213,8 +213,7 @@ void Groth16Prover::release() noexcept
     if (ev_start_) (void)hipEventDestroy(ev_start_);
     if (ev_wsched_) (void)hipEventDestroy(ev_wsched_);
     ev_start_ = ev_wsched_ = nullptr;
-    if (pinned_witness_) (void)hipHostFree(pinned_witness_);
-    pinned_witness_ = nullptr;
+    upload_.reset();
 }
 
 uint64_t Groth16Prover::device_bytes() const
@@ -222,6 +221,7 @@ uint64_t Groth16Prover::device_bytes() const
     uint64_t b = a_.bytes() + b1_.bytes() + l_.bytes() + h_.bytes() + b2_.bytes();
     for (int m = 0; m < 3; ++m) b += row_ptr_[m].bytes() + col_idx_[m].bytes() + values_[m].bytes();
     b += z_.bytes() + pa_.bytes() + pb_.bytes() + pc_.bytes();
+    if (upload_) b += upload_->device_bytes();
     return b;
 }
 
@@ -229,9 +229,7 @@ void Groth16Prover::prove(const void *witness, void *raw_out, void *az, void *bz
 {
     NKR_REQUIRE(witness && raw_out, "null witness or output");
     NKR_HIP(hipSetDevice(device_));
-    const size_t fr_bytes = (size_t)cols_ * sizeof(fr_t);
-    std::memcpy(pinned_witness_, witness, fr_bytes);
-    NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, stream_));
+    upload_->upload(static_cast<const fr_t *>(witness), z_.get(), stream_);
     run(z_.get(), raw_out, az, bz, cz, h);
 }
 
@@ -263,9 +261,7 @@ void Groth16Prover::prove_blind(const void *witness, bool on_device, const fr_t 
     NKR_HIP(hipSetDevice(device_));
     const fr_t *z = static_cast<const fr_t *>(witness);
     if (!on_device) {
-        const size_t fr_bytes = (size_t)cols_ * sizeof(fr_t);
-        std::memcpy(pinned_witness_, witness, fr_bytes);
-        NKR_HIP(hipMemcpyAsync(z_.get(), pinned_witness_, fr_bytes, hipMemcpyHostToDevice, stream_));
+        upload_->upload(z, z_.get(), stream_);
         z = z_.get();
     }
     const BlindJob job{r, s, proof_out};

@@ -97,6 +97,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_profile": [vp, i32],
             "nkr_g16_profile_read": [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64)],
             "nkr_dev_copy": [vp, vp, u64, vp],
+            "nkr_valu_probe": [i32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
             "nkr_ntt_set_shape": [u32, u32, u32],
             "nkr_msm_set_shape": [u32, u32],
             "nkr_g16_quotient": [vp, vp, vp, u64, vp, i32],

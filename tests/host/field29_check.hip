@@ -1,0 +1,147 @@
+// CPU check of the reduced-radix field and its mixed addition
+// (nkr-zkp_amd/csrc/field29.h, curve29.h) against the canonical 32-bit-limb
+// code (field.h, curve.h), compiled as host code.
+//   * conversions: canonical -> 29-bit -> canonical is the identity; the
+//     29-bit Montgomery one converts to the canonical one;
+//   * products: every input bound the formulas use (below 2p, 10p, and 32p
+//     against 2p), built as x + k*p, gives fp_mul's canonical result;
+//   * lazy sums and differences: a + b and a + K*p - b are congruent to the
+//     canonical ones;
+//   * mixed addition: chains of xyzz29_add_affine over random bases, repeated
+//     bases (doubling), negated ones (infinity) and infinity restarts give,
+//     after every step, the canonical coordinates of xyzz_add_affine.
+// Prints "ok <checks>" or the first mismatches; exit status 1 on a mismatch.
+#include <cstdio>
+#include <vector>
+
+#include "curve29.h"
+
+using namespace nkr;
+
+static uint64_t g_state = 0x9E3779B97F4A7C15ull;
+static uint32_t rnd()
+{
+    g_state ^= g_state << 13;
+    g_state ^= g_state >> 7;
+    g_state ^= g_state << 17;
+    return (uint32_t)(g_state >> 11);
+}
+
+template <class P>
+static Fp<P> rand_fp()
+{
+    Fp<P> x;
+    for (int i = 0; i < 8; ++i) x.l[i] = rnd();
+    x.l[7] %= P::P[7] + 1;  // below 2p
+    return fp_reduce_once(x);
+}
+
+static int g_bad = 0;
+static long g_doublings = 0, g_cancels = 0;
+static long g_checks = 0;
+static void expect(bool ok, const char *what, int i)
+{
+    ++g_checks;
+    if (!ok && g_bad++ < 10) std::printf("mismatch: %s at %d\n", what, i);
+}
+
+// x + K*p in the 29-bit form, x canonical (value below (K + 2) p)
+template <class P, uint32_t K>
+static Fp29<P> widen(const Fp<P> &x)
+{
+    return fp29_sub<P, K>(fp29_reduce_to_2p(fp29_from_mont(x)), Fp29<P>::zero());
+}
+
+template <class P>
+static void check_field(int n)
+{
+    for (int i = 0; i < n; ++i) {
+        Fp<P> x = rand_fp<P>(), y = rand_fp<P>();
+        if (i == 0) x = y = Fp<P>::zero();
+        if (i == 1) x = y = Fp<P>::one();
+        if (i == 2) {
+            x = Fp<P>::modulus();
+            x.l[0] -= 1;  // p - 1
+            y = x;
+        }
+        const Fp<P> want = fp_mul(x, y);
+        expect(fp29_to_mont(fp29_from_mont(x)) == x, "round trip", i);
+        expect(fp29_to_mont(fp29_mul(fp29_from_mont(x), fp29_reduce_to_2p(fp29_from_mont(y)))) == want,
+               "mul 32p x 2p", i);
+        expect(fp29_to_mont(fp29_mul(widen<P, 8>(x), widen<P, 8>(y))) == want, "mul 10p x 10p", i);
+        expect(fp29_to_mont(fp29_mul(widen<P, 0>(x), widen<P, 0>(y))) == want, "mul 2p x 2p", i);
+        const Fp29<P> m = fp29_mul(widen<P, 8>(x), widen<P, 8>(y));
+        bool limbs = true;
+        for (int k = 0; k < 9; ++k) limbs = limbs && (m.l[k] >> 29) == 0;
+        expect(limbs, "product limbs below 2^29", i);
+        expect(fp29_to_mont(fp29_add(widen<P, 0>(x), widen<P, 4>(y))) == fp_add(x, y), "add", i);
+        expect(fp29_to_mont(fp29_sub<P, 8>(widen<P, 0>(x), widen<P, 4>(y))) == fp_sub(x, y), "sub", i);
+        expect(fp29_to_mont(fp29_reduce_to_8p(fp29_from_mont(x))) == x, "reduce to 8p", i);
+        const bool z = fp_sub(x, y).is_zero();
+        const Fp29<P> d = fp29_sub<P, 8>(widen<P, 0>(x), widen<P, 0>(y));
+        expect(!z || fp29_may_be_zero<P, 10>(d), "zero filter", i);
+    }
+    expect(fp29_to_mont(Fp29<P>::one()) == Fp<P>::one(), "one", 0);
+    // the zero filter passes every multiple j*p below 10p
+    const Fp29<P> z = Fp29<P>::zero();
+    expect(fp29_may_be_zero<P, 10>(z) && fp29_may_be_zero<P, 10>(fp29_sub<P, 1>(z, z)) &&
+               fp29_may_be_zero<P, 10>(fp29_sub<P, 5>(z, z)) && fp29_may_be_zero<P, 10>(fp29_sub<P, 9>(z, z)),
+           "zero filter on multiples of p", 0);
+}
+
+static Affine<fq_t> random_base(const Affine<fq_t> &g)
+{
+    uint32_t k[8];
+    for (int i = 0; i < 8; ++i) k[i] = rnd();
+    k[7] &= 0x0fffffffu;
+    return xyzz_to_affine(xyzz_mul_u256(Xyzz<fq_t>::from_affine(g), k));
+}
+
+static void check_madd(int chains, int len)
+{
+    Affine<fq_t> g;
+    g.x = fp_to_mont(fq_t::zero());
+    g.x.l[0] = 1;
+    g.x = fp_to_mont(g.x);
+    g.y = fq_t::zero();
+    g.y.l[0] = 2;
+    g.y = fp_to_mont(g.y);
+    std::vector<Affine<fq_t>> pool;
+    for (int i = 0; i < 24; ++i) pool.push_back(random_base(g));
+    for (int c = 0; c < chains; ++c) {
+        Xyzz<fq_t> a32 = Xyzz<fq_t>::infinity();
+        Xyzz29<FqParams> a29 = Xyzz29<FqParams>::infinity();
+        for (int e = 0; e < len; ++e) {
+            const uint32_t r = rnd();
+            Affine<fq_t> b = pool[r % pool.size()];
+            if ((r >> 8) % 5 == 0 && !a32.is_inf()) {  // doubling
+                b = xyzz_to_affine(a32);
+                ++g_doublings;
+            }
+            if ((r >> 12) % 7 == 0) b.y = fp_neg(b.y);
+            if ((r >> 16) % 11 == 0 && !a32.is_inf()) {  // back to infinity
+                b = xyzz_to_affine(a32);
+                b.y = fp_neg(b.y);
+                ++g_cancels;
+            }
+            a32 = xyzz_add_affine(a32, b.x, b.y);
+            a29 = xyzz29_add_affine(a29, fp29_from_mont(b.x), fp29_from_mont(b.y));
+            const Xyzz<fq_t> got = xyzz29_to(a29);
+            expect(got.X == a32.X && got.Y == a32.Y && got.ZZ == a32.ZZ && got.ZZZ == a32.ZZZ, "madd chain",
+                   c * len + e);
+        }
+    }
+}
+
+int main()
+{
+    check_field<FqParams>(20000);
+    check_field<FrParams>(20000);
+    check_madd(200, 80);
+    if (g_bad) {
+        std::printf("FAILED %d of %ld checks\n", g_bad, g_checks);
+        return 1;
+    }
+    std::printf("ok %ld checks (%ld doublings, %ld cancellations)\n", g_checks, g_doublings, g_cancels);
+    return 0;
+}

@@ -28,10 +28,12 @@ def test_more_than_one_gpu_is_refused():
 
 
 def test_measured_traffic_reads_the_committed_summary():
-    t = bench.measured_traffic()
-    assert isinstance(t, int) and t > 0
     with open(bench.TRAFFIC_PROFILE) as f:
-        assert f"traffic_bytes_per_prove: {t}" in f.read()
+        text = f.read()
+    for kernel in ("g1", "g2"):
+        t = bench.measured_traffic(kernel)
+        assert isinstance(t, int) and t > 0
+        assert f"traffic_bytes_{kernel}_accumulate: {t}" in text
 
 
 def test_montgomery_one():
