@@ -7,11 +7,11 @@
 // libff/algebra/fields/fp.hpp:42), so buffers move between host and device
 // without conversion.
 //
-// Multiplication is Montgomery over 32-bit words.  Both BN254 moduli have a
-// top word below 2^30, so partial results stay below 2p < 2^255 and never
-// spill into a ninth word.  Each 32x32+64 step maps onto one v_mad_u64_u32
-// (full rate on gfx950); the integer product is not a dense contraction, so
-// nothing here uses MFMA.
+// Multiplication is Montgomery with the operands regrouped into 29-bit limbs
+// (fp_mul_r29 below): each 29x29+64 step is one v_mad_u64_u32 with no carry
+// to propagate; additions and subtractions stay on the 32-bit limbs, whose
+// carries the hardware chains.  The integer product is not a dense
+// contraction, so nothing here uses MFMA.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -173,166 +173,259 @@ NKR_HD Fp<P> fp_neg(const Fp<P> &a)
     return d;
 }
 
-// Montgomery product a*b*2^-256 mod p, inputs and output < p (word-serial
-// CIOS in plain C; the host build of this header uses it).
+// ---------------------------------------------------------------------------
+// Reduced-radix products.  An element of the 29-bit form is 9 limbs of 29 bits
+// (value = sum l_k 2^(29k), room for 261 bits) holding a Montgomery residue
+// with R = 2^261.  With 32-bit limbs every 32x32+64 multiply-add of a
+// Montgomery product can overflow its 64-bit accumulator, so each
+// v_mad_u64_u32 needs a carry add behind it, and the carry travels through
+// an SGPR, which costs wait states.  With 29-bit limbs a product term is
+// below 2^58 and a column of at most 18 terms plus the carry-in stays below
+// 2^63: one v_mad_u64_u32 per term and no carries inside a column (162
+// multiply-adds and ~90 other instructions per product).  The canonical
+// product below (fp_mul) goes through it: x*y*2^-256 = (32x)*y*2^-261, the
+// shift by 5 being free in the limb conversion.  Measured on MI355X in
+// chains of dependent products: 126.5 G canonical products/s against 106.6 G
+// for the 32-bit product-scanning form (field29.h builds lazily reduced
+// arithmetic on the same form for the MSM accumulation).
+
+constexpr uint32_t kMask29 = (1u << 29) - 1;
+
+// Limbs of k * p in the 29-bit radix, computed at compile time (k * p < 2^261).
 template <class P>
-NKR_HD Fp<P> fp_mul_cios(const Fp<P> &a, const Fp<P> &b)
-{
-    uint32_t t[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) t[j] = 0;
-
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint32_t bi = b.l[i];
-        uint64_t s = (uint64_t)a.l[0] * bi + t[0];
-        uint32_t A = (uint32_t)(s >> 32);
-        const uint32_t t0 = (uint32_t)s;
-        const uint32_t m = t0 * P::INV;
-        uint64_t u = (uint64_t)m * P::P[0] + t0;
-        uint32_t C = (uint32_t)(u >> 32);
-#pragma unroll
-        for (int j = 1; j < 8; ++j) {
-            s = (uint64_t)a.l[j] * bi + t[j] + A;
-            A = (uint32_t)(s >> 32);
-            u = (uint64_t)m * P::P[j] + (uint32_t)s + C;
-            C = (uint32_t)(u >> 32);
-            t[j - 1] = (uint32_t)u;
-        }
-        t[7] = C + A;
+struct P29 {
+    struct Limbs {
+        uint32_t l[9];
+    };
+    static constexpr uint32_t bits(int k)
+    {
+        const int lo = 29 * k, wi = lo / 32, sh = lo % 32;
+        uint64_t v = wi < 8 ? ((uint64_t)P::P[wi] >> sh) : 0;
+        if (wi + 1 < 8 && sh > 3) v |= (uint64_t)P::P[wi + 1] << (32 - sh);
+        return (uint32_t)v & kMask29;
     }
-    Fp<P> r;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.l[j] = t[j];
-    return fp_reduce_once(r);
-}
-
-#if defined(__HIP_DEVICE_COMPILE__)
-// acc(96 bit) += x*y.  One v_mad_u64_u32 into the low 64 bits; its carry-out
-// (an SGPR lane mask) feeds v_addc_co_u32 on the top word.
-NKR_DEV void mac96(uint64_t &acc, uint32_t &top, uint32_t x, uint32_t y)
-{
-    uint64_t cc;
-    asm("v_mad_u64_u32 %0, %2, %3, %4, %0\n\t"
-        "v_addc_co_u32 %1, %2, %1, 0, %2"
-        : "+v"(acc), "+v"(top), "=&s"(cc)
-        : "v"(x), "v"(y));
-}
-
-// Montgomery product by product scanning (column-wise, "FIPS" order): column
-// k accumulates a_j*b_{k-j} and m_j*p_{k-j} into a 96-bit accumulator, so each
-// partial product costs a v_mad_u64_u32 plus one carry add and no moves.
-//
-// With kReduce = false the final subtraction is skipped: for a*b < 4p^2
-// (say a < 4p and b < p, or both below 2p) and 4p < 2^256 the result is
-// below 2p, which lazily reduced transforms keep as their invariant.
-template <class P, bool kReduce = true>
-NKR_DEV Fp<P> fp_mul_fips(const Fp<P> &a, const Fp<P> &b)
-{
-    uint32_t m[8];
-    uint32_t t[8];
-    uint64_t acc = 0;
-    uint32_t top = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-#pragma unroll
-        for (int j = 0; j < i; ++j) {
-            mac96(acc, top, a.l[j], b.l[i - j]);
-            mac96(acc, top, m[j], P::P[i - j]);
+    static constexpr Limbs times(uint32_t k)
+    {
+        Limbs r{};
+        uint64_t carry = 0;
+        for (int i = 0; i < 9; ++i) {
+            const uint64_t t = (uint64_t)k * bits(i) + carry;
+            r.l[i] = (uint32_t)t & kMask29;
+            carry = t >> 29;
         }
-        mac96(acc, top, a.l[i], b.l[0]);
-        m[i] = (uint32_t)acc * P::INV;
-        mac96(acc, top, m[i], P::P[0]);
-        acc = (acc >> 32) | ((uint64_t)top << 32);
-        top = 0;
-    }
-#pragma unroll
-    for (int i = 8; i < 15; ++i) {
-#pragma unroll
-        for (int j = i - 7; j < 8; ++j) {
-            mac96(acc, top, a.l[j], b.l[i - j]);
-            mac96(acc, top, m[j], P::P[i - j]);
-        }
-        t[i - 8] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)top << 32);
-        top = 0;
-    }
-    // (a*b + m*p) / 2^256 < 2p < 2^255, so the last column fits one word.
-    t[7] = (uint32_t)acc;
-    Fp<P> r;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r.l[j] = t[j];
-    if constexpr (kReduce)
-        return fp_reduce_once(r);
-    else
         return r;
-}
+    }
+    // -p^-1 mod 2^29
+    static constexpr uint32_t inv() { return P::INV & kMask29; }
+    // 2^261 mod p, the Montgomery form of 1: (2^256 mod p) * 32 reduced.
+    static constexpr Limbs one()
+    {
+        Limbs v{};
+        // ONE (< p, 8 x 32 bits) shifted left by 5 into 29-bit limbs
+        for (int k = 0; k < 9; ++k) {
+            const int lo = 29 * k - 5;
+            uint64_t x = 0;
+            if (lo < 0) {
+                x = (uint64_t)P::ONE[0] << (-lo);
+            } else {
+                const int wi = lo / 32, sh = lo % 32;
+                x = wi < 8 ? ((uint64_t)P::ONE[wi] >> sh) : 0;
+                if (wi + 1 < 8 && sh > 0) x |= (uint64_t)P::ONE[wi + 1] << (32 - sh);
+            }
+            v.l[k] = (uint32_t)x & kMask29;
+        }
+        for (uint32_t m = 16; m >= 1; m >>= 1) {  // subtract 16p, 8p, 4p, 2p, p where possible
+            const Limbs mp = times(m);
+            Limbs d{};
+            int64_t borrow = 0;
+            for (int i = 0; i < 9; ++i) {
+                const int64_t t = (int64_t)v.l[i] - mp.l[i] - borrow;
+                d.l[i] = (uint32_t)(t & kMask29);
+                borrow = t < 0 ? 1 : 0;
+            }
+            if (!borrow) v = d;
+        }
+        return v;
+    }
+};
 
-// Two independent mac96 steps in one block: each v_mad_u64_u32 has the
-// other chain's instruction beside it while its result is pending, so one
-// wave keeps issuing where a single chain would wait on the multiplier.
-NKR_DEV void mac96x2(uint64_t &acc0, uint32_t &top0, uint32_t x0, uint32_t y0, uint64_t &acc1, uint32_t &top1,
-                     uint32_t x1, uint32_t y1)
-{
-    uint64_t c0, c1;
-    asm("v_mad_u64_u32 %0, %4, %6, %7, %0\n\t"
-        "v_mad_u64_u32 %2, %5, %8, %9, %2\n\t"
-        "v_addc_co_u32 %1, %4, %1, 0, %4\n\t"
-        "v_addc_co_u32 %3, %5, %3, 0, %5"
-        : "+v"(acc0), "+v"(top0), "+v"(acc1), "+v"(top1), "=&s"(c0), "=&s"(c1)
-        : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
-}
-
-// Two independent Montgomery products (r0 = a0*b0, r1 = a1*b1) with their
-// column steps interleaved: the same instructions as two fp_mul_fips calls,
-// in half the dependent-chain latency.  Outputs may alias inputs.
 template <class P>
-NKR_DEV void fp_mul_fips2(const Fp<P> &a0, const Fp<P> &b0, const Fp<P> &a1, const Fp<P> &b1, Fp<P> &r0,
-                          Fp<P> &r1)
+struct Fp29 {
+    uint32_t l[9];
+
+    NKR_HD static Fp29 zero()
+    {
+        Fp29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = 0;
+        return r;
+    }
+    NKR_HD static Fp29 one()
+    {
+        constexpr typename P29<P>::Limbs o = P29<P>::one();
+        Fp29 r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) r.l[i] = o.l[i];
+        return r;
+    }
+    // all limbs zero (the exact zero this code writes for the point at infinity)
+    NKR_HD bool is_exact_zero() const
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc |= l[i];
+        return acc == 0;
+    }
+};
+
+// x*2^256 (canonical, 8 x 32 bits) -> x*2^261 as 29-bit limbs: a 5-bit shift,
+// no reduction; the value is below 32p.
+template <class P>
+NKR_HD Fp29<P> fp29_from_mont(const Fp<P> &x)
 {
-    uint32_t m0[8], m1[8], t0[8], t1[8];
-    uint64_t acc0 = 0, acc1 = 0;
-    uint32_t top0 = 0, top1 = 0;
+    Fp29<P> r;
+    r.l[0] = (x.l[0] << 5) & kMask29;
+#pragma unroll
+    for (int k = 1; k < 9; ++k) {
+        const int lo = 29 * k - 5, wi = lo / 32, sh = lo % 32;
+        uint32_t v = x.l[wi] >> sh;
+        if (wi + 1 < 8 && sh > 3) v |= x.l[wi + 1] << (32 - sh);
+        r.l[k] = v & kMask29;
+    }
+    return r;
+}
+
+// The bits of x (any 256-bit value) as 29-bit limbs, value unchanged.
+template <class P>
+NKR_HD Fp29<P> fp29_from_bits(const Fp<P> &x)
+{
+    Fp29<P> r;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int lo = 29 * k, wi = lo / 32, sh = lo % 32;
+        uint32_t v = x.l[wi] >> sh;
+        if (wi + 1 < 8 && sh > 3) v |= x.l[wi + 1] << (32 - sh);
+        r.l[k] = v & kMask29;
+    }
+    return r;
+}
+
+// The value of x (below 2^256) as 8 x 32-bit limbs.
+template <class P>
+NKR_HD Fp<P> fp29_to_bits(const Fp29<P> &x)
+{
+    Fp<P> r;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-#pragma unroll
-        for (int j = 0; j < i; ++j) {
-            mac96x2(acc0, top0, a0.l[j], b0.l[i - j], acc1, top1, a1.l[j], b1.l[i - j]);
-            mac96x2(acc0, top0, m0[j], P::P[i - j], acc1, top1, m1[j], P::P[i - j]);
-        }
-        mac96x2(acc0, top0, a0.l[i], b0.l[0], acc1, top1, a1.l[i], b1.l[0]);
-        m0[i] = (uint32_t)acc0 * P::INV;
-        m1[i] = (uint32_t)acc1 * P::INV;
-        mac96x2(acc0, top0, m0[i], P::P[0], acc1, top1, m1[i], P::P[0]);
-        acc0 = (acc0 >> 32) | ((uint64_t)top0 << 32);
-        acc1 = (acc1 >> 32) | ((uint64_t)top1 << 32);
-        top0 = top1 = 0;
+        const int lo = 32 * i, k = lo / 29, sh = lo % 29;
+        uint32_t w = x.l[k] >> sh;
+        if (k + 1 < 9) w |= x.l[k + 1] << (29 - sh);
+        if (k + 2 < 9 && 58 - sh < 32) w |= x.l[k + 2] << (58 - sh);
+        r.l[i] = w;
     }
-#pragma unroll
-    for (int i = 8; i < 15; ++i) {
-#pragma unroll
-        for (int j = i - 7; j < 8; ++j) {
-            mac96x2(acc0, top0, a0.l[j], b0.l[i - j], acc1, top1, a1.l[j], b1.l[i - j]);
-            mac96x2(acc0, top0, m0[j], P::P[i - j], acc1, top1, m1[j], P::P[i - j]);
-        }
-        t0[i - 8] = (uint32_t)acc0;
-        t1[i - 8] = (uint32_t)acc1;
-        acc0 = (acc0 >> 32) | ((uint64_t)top0 << 32);
-        acc1 = (acc1 >> 32) | ((uint64_t)top1 << 32);
-        top0 = top1 = 0;
-    }
-    t0[7] = (uint32_t)acc0;
-    t1[7] = (uint32_t)acc1;
-    Fp<P> x0, x1;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        x0.l[j] = t0[j];
-        x1.l[j] = t1[j];
-    }
-    r0 = fp_reduce_once(x0);
-    r1 = fp_reduce_once(x1);
+    return r;
 }
-#endif
+
+// Montgomery product a*b*2^-261 mod p by product scanning.  Inputs: limbs
+// below 2^29 and a*b < 2^261 * p (e.g. both below 10p, or a < 32p and
+// b < 2p).  Output: limbs below 2^29, value below p + a*b/2^261 < 2p.
+template <class P>
+NKR_HD Fp29<P> fp29_mul(const Fp29<P> &a, const Fp29<P> &b)
+{
+    constexpr uint32_t INV = P29<P>::inv();
+    uint32_t m[9];
+    Fp29<P> r;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        m[k] = ((uint32_t)acc * INV) & kMask29;
+        acc += (uint64_t)m[k] * P29<P>::bits(0);
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        r.l[k - 9] = (uint32_t)acc & kMask29;
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
+// Two independent products r0 = a0*b0 and r1 = a1*b1 (fp29_mul's bounds for
+// each) with their multiply-adds interleaved: each product's column sum is
+// one dependent chain through its accumulator, so a wave that alternates two
+// chains issues while the other's v_mad_u64_u32 is still in flight.  The same
+// results as two fp29_mul calls.  Outputs may alias inputs.
+template <class P>
+NKR_HD void fp29_mul2(const Fp29<P> &a0, const Fp29<P> &b0, const Fp29<P> &a1, const Fp29<P> &b1, Fp29<P> &r0,
+                      Fp29<P> &r1)
+{
+    constexpr uint32_t INV = P29<P>::inv();
+    uint32_t m0[9], m1[9], t0[9], t1[9];
+    uint64_t acc0 = 0, acc1 = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) {
+            acc0 += (uint64_t)a0.l[i] * b0.l[k - i];
+            acc1 += (uint64_t)a1.l[i] * b1.l[k - i];
+        }
+#pragma unroll
+        for (int i = 0; i < k; ++i) {
+            acc0 += (uint64_t)m0[i] * P29<P>::bits(k - i);
+            acc1 += (uint64_t)m1[i] * P29<P>::bits(k - i);
+        }
+        m0[k] = ((uint32_t)acc0 * INV) & kMask29;
+        m1[k] = ((uint32_t)acc1 * INV) & kMask29;
+        acc0 += (uint64_t)m0[k] * P29<P>::bits(0);
+        acc1 += (uint64_t)m1[k] * P29<P>::bits(0);
+        acc0 >>= 29;
+        acc1 >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) {
+            acc0 += (uint64_t)a0.l[i] * b0.l[k - i];
+            acc1 += (uint64_t)a1.l[i] * b1.l[k - i];
+        }
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) {
+            acc0 += (uint64_t)m0[i] * P29<P>::bits(k - i);
+            acc1 += (uint64_t)m1[i] * P29<P>::bits(k - i);
+        }
+        t0[k - 9] = (uint32_t)acc0 & kMask29;
+        t1[k - 9] = (uint32_t)acc1 & kMask29;
+        acc0 >>= 29;
+        acc1 >>= 29;
+    }
+    t0[8] = (uint32_t)acc0;
+    t1[8] = (uint32_t)acc1;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        r0.l[i] = t0[i];
+        r1.l[i] = t1[i];
+    }
+}
+
+// x*w*2^-256 mod p (the canonical form's Montgomery product) through the
+// 29-bit product: (32x)*w*2^-261.  x: any 256-bit value; w: below p.  The
+// result is below p + x*w/2^256 < 2p, not reduced, in 32-bit limbs.
+template <class P>
+NKR_HD Fp<P> fp_mul_r29(const Fp<P> &x, const Fp<P> &w)
+{
+    return fp29_to_bits(fp29_mul(fp29_from_mont(x), fp29_from_bits(w)));
+}
+
+
 
 #if !defined(__HIP_DEVICE_COMPILE__)
 // Host build: the same no-carry CIOS over 4 x 64-bit words with 128-bit
@@ -366,14 +459,14 @@ inline Fp<P> fp_mul_host64(const Fp<P> &a, const Fp<P> &b)
 }
 #endif
 
-// Montgomery product a*b*2^-256 mod p.  On gfx950 the product-scanning form
-// issues ~300 VALU instructions against ~880 for the C form (measured 129 vs
-// 100 G products/s on MI355X); the host build uses 64-bit words.
+// Montgomery product a*b*2^-256 mod p, inputs and output below p.  On gfx950
+// through the 29-bit product (fp_mul_r29) and one conditional subtraction;
+// the host build uses 64-bit words.
 template <class P>
 NKR_HD Fp<P> fp_mul(const Fp<P> &a, const Fp<P> &b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return fp_mul_fips(a, b);
+    return fp_reduce_once(fp_mul_r29(a, b));
 #else
     return fp_mul_host64(a, b);
 #endif
@@ -384,7 +477,10 @@ template <class P>
 NKR_HD void fp_mul2(const Fp<P> &a0, const Fp<P> &b0, const Fp<P> &a1, const Fp<P> &b1, Fp<P> &r0, Fp<P> &r1)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    fp_mul_fips2(a0, b0, a1, b1, r0, r1);
+    Fp29<P> x, y;
+    fp29_mul2(fp29_from_mont(a0), fp29_from_bits(b0), fp29_from_mont(a1), fp29_from_bits(b1), x, y);
+    r0 = fp_reduce_once(fp29_to_bits(x));
+    r1 = fp_reduce_once(fp29_to_bits(y));
 #else
     const Fp<P> x = fp_mul_host64(a0, b0), y = fp_mul_host64(a1, b1);
     r0 = x;

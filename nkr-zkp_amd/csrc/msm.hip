@@ -65,8 +65,14 @@ __device__ __forceinline__ void for_each_digit(const fr_t &s, uint32_t c, uint32
     for (uint32_t w = 0; w < W; ++w) {
         const uint32_t off = w * c;
         const uint32_t word = off >> 5, sh = off & 31u;
-        const uint64_t lo = s.l[word];
-        const uint64_t hi = (word + 1 < 8) ? s.l[word + 1] : 0u;
+        // selects instead of s.l[word]: a dynamically indexed register array
+        // would live in scratch memory
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            lo = k == word ? s.l[k] : lo;
+            hi = k == word + 1 ? s.l[k] : hi;
+        }
         uint32_t v = (uint32_t)(((hi << 32) | lo) >> sh) & mask;
         v += carry;
         int32_t d;
@@ -441,26 +447,21 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
             // G1: the accumulator stays in the reduced-radix form
             // (curve29.h: one v_mad_u64_u32 per product term, no carries)
             // for the whole piece; bases are converted as they arrive and
-            // the partial sum once at the end.  Software pipeline: the
-            // gather of entry e + 1's base is issued before entry e's
-            // addition, so its latency hides behind the field arithmetic.
+            // the partial sum once at the end.  The next sorted index is
+            // fetched one entry ahead; the base itself is not (its 16
+            // registers cost more occupancy than its latency costs three
+            // resident waves, measured on MI355X).
             Xyzz29<FqParams> acc29 = Xyzz29<FqParams>::infinity();
             uint32_t v = sorted[beg];
-            Affine<F> P = bases[v & kEntryMask];
 #pragma unroll 1
             for (uint32_t e = beg; e < end; ++e) {
-                uint32_t vn = 0;
-                Affine<F> Pn;
-                if (e + 1 < end) {
-                    vn = sorted[e + 1];
-                    Pn = bases[vn & kEntryMask];
-                }
+                const Affine<F> P = bases[v & kEntryMask];
+                const uint32_t vn = e + 1 < end ? sorted[e + 1] : 0u;
                 if (!affine_is_inf(P)) {
                     const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
                     acc29 = xyzz29_add_affine(acc29, fp29_from_mont(P.x), fp29_from_mont(y));
                 }
                 v = vn;
-                P = Pn;
             }
             acc = xyzz29_to(acc29);
         }

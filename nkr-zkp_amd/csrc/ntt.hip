@@ -119,14 +119,11 @@ __device__ __forceinline__ fr_t sub_lazy(const fr_t &x, const fr_t &y)
 template <bool DIT>
 constexpr bool kLazyNtt = !DIT;
 
-// x * w * 2^-256 for x < 4r and a canonical w: below 2r.
+// x * w * 2^-256 for x < 4r and a canonical w: below r + x*w/2^256 < 2r
+// (the 29-bit product without its final subtraction).
 __device__ __forceinline__ fr_t mul_lazy(const fr_t &x, const fr_t &w)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return fp_mul_fips<FrParams, false>(x, w);
-#else
-    return fp_mul(x, w);
-#endif
+    return fp_mul_r29(x, w);
 }
 
 // LDS tile of n field elements: two planes of 16-byte halves (element i's
