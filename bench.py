@@ -315,8 +315,8 @@ def main() -> int:
         return 2
 
     if args.circuit == "poseidon":
-        log(f"generating {args.hashes} folded Poseidon-style hashes seed={SEED:#x} (about 80 s for 4096)")
-        circ = circuits.poseidon(args.hashes, SEED, folded=True)
+        log(f"generating {args.hashes} folded Poseidon-style hashes seed={SEED:#x}")
+        circ = circuits.generate("poseidon", args.hashes, SEED, folded=True)
         workload = f"groth16_prove_poseidon{args.hashes}_folded"
     else:
         log(f"generating matmul {args.size} ({args.density}) seed={SEED:#x}")

@@ -414,8 +414,8 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const JobDev<F> *__restrict
 // bucket into one partial, then moves on by the grid stride, so the launch
 // can be sized to the resident waves and every wave gets the same share.
 template <class F>
-__global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
-                                                    const uint32_t *__restrict__ piece_base, uint32_t J)
+__device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jobs,
+                                                const uint32_t *__restrict__ piece_base, uint32_t J)
 {
     const uint32_t total = piece_base[J];
 #pragma unroll 1
@@ -467,6 +467,24 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
         }
         jb.pool[p] = acc;
     }
+}
+
+template <class F>
+__global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ piece_base, uint32_t J)
+{
+    accumulate_body<F>(jobs, piece_base, J);
+}
+
+// The G2 addition's working set exceeds 256 registers, which would leave one
+// wave per SIMD; held to two waves, its spills to scratch cost less than the
+// second wave gains (G2 accumulation of the dense 2^20 Poseidon-style proof
+// 10.3 -> 8.5 ms on MI355X).
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_accumulate<fq2_t>(
+    const JobDev<fq2_t> *__restrict__ jobs, const uint32_t *__restrict__ piece_base, uint32_t J)
+{
+    accumulate_body<fq2_t>(jobs, piece_base, J);
 }
 
 // Fold level L: buckets with more than kMsmCompleteMax partials get
