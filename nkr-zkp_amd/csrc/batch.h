@@ -7,11 +7,15 @@
 //   SpMV        one launch over (job, matrix, row)
 //   NTT chain   one launch per pass per size class (all A, B, C of the class)
 //   quotient    one launch per size class
-//   sort        one launch per sort phase for all witness and H schedules
+//   sort        one launch per sort phase for all witness schedules, and
+//               one for all H schedules
 //   MSM         one MsmBatch for all G1 MSMs (4 per proof) and one for the
 //               G2 MSMs: each accumulation / fold / completion / tree level is
 //               one launch for the whole batch
-// All on one stream: each stage already fills the device.
+// Two streams, as in the plain prover: the witness schedules and the G2 MSMs
+// need only the witnesses, so they run beside the SpMV, transforms, H
+// schedules and G1 MSMs; each side's latency-bound reduction levels then
+// overlap the other side's accumulation.
 //   host tail   the r,s-only blinding terms while the GPU works, then the
 //               final sums and blinding on a pool of host threads.
 // Each proof's result is bit-identical to proving it alone.
@@ -50,6 +54,7 @@ public:
     int device() const { return device_; }
 
 private:
+    void release() noexcept;  // streams and events (destructor, failed constructor)
     struct Slot {
         DeviceBuffer<fr_t> z, pa, pb, pc;
         MsmSchedule wsched, hsched;
@@ -57,9 +62,11 @@ private:
         DeviceBuffer<Xyzz<fq2_t>> planes2;
     };
     int device_;
-    hipStream_t stream_ = nullptr;
+    hipStream_t stream_ = nullptr;   // SpMV, transforms, H schedules, G1 MSMs
+    hipStream_t stream2_ = nullptr;  // witness schedules and G2 MSMs, beside them
+    hipEvent_t ev_z_ = nullptr, ev_wsched_ = nullptr, ev_g2_ = nullptr;
     std::vector<std::unique_ptr<Slot>> slots_;
-    MsmScheduleBatch sort_batch_;
+    MsmScheduleBatch sort_w_, sort_h_;
     MsmBatch<fq_t> batch_g1_;
     MsmBatch<fq2_t> batch_g2_;
     PinnedStaging<uint8_t> witness_stage_, table_stage_;

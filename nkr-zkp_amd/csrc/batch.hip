@@ -50,15 +50,40 @@ MsmPlan batch_plan(const MsmPlan &plan) { return plan.with_piece(kBatchPiece); }
 BatchProver::BatchProver(int device) : device_(device)
 {
     NKR_HIP(hipSetDevice(device_));
+    struct Undo {
+        BatchProver *p;
+        bool armed = true;
+        ~Undo()
+        {
+            if (armed) p->release();
+        }
+    } undo{this};
     NKR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+    NKR_HIP(hipEventCreateWithFlags(&ev_z_, hipEventDisableTiming));
+    NKR_HIP(hipEventCreateWithFlags(&ev_wsched_, hipEventDisableTiming));
+    NKR_HIP(hipEventCreateWithFlags(&ev_g2_, hipEventDisableTiming));
+    undo.armed = false;
 }
 
 BatchProver::~BatchProver()
 {
-    if (stream_) {
-        (void)hipStreamSynchronize(stream_);
-        (void)hipStreamDestroy(stream_);
-    }
+    release();
+}
+
+void BatchProver::release() noexcept
+{
+    for (hipStream_t *s : {&stream2_, &stream_})
+        if (*s) {
+            (void)hipStreamSynchronize(*s);
+            (void)hipStreamDestroy(*s);
+            *s = nullptr;
+        }
+    for (hipEvent_t *e : {&ev_z_, &ev_wsched_, &ev_g2_})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
 }
 
 void BatchProver::prove(const std::vector<BatchItem> &items)
@@ -102,6 +127,23 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         NKR_HIP(hipMemcpyAsync(sl.z.get(), wst + woff[i], woff[i + 1] - woff[i], hipMemcpyHostToDevice, stream_));
     }
     witness_stage_.uploaded(stream_);
+
+    // Side stream: the witness schedules and the G2 MSMs need only z.
+    NKR_HIP(hipEventRecord(ev_z_, stream_));
+    NKR_HIP(hipStreamWaitEvent(stream2_, ev_z_, 0));
+    {
+        std::vector<MsmScheduleBatch::Item> wsorts;
+        std::vector<MsmJob<fq2_t>> g2jobs;
+        for (size_t i = 0; i < K; ++i) {
+            Groth16Prover &g = *items[i].ctx;
+            Slot &sl = *slots_[i];
+            wsorts.push_back({&sl.wsched, sl.z.get(), batch_plan(g.wplan_)});
+            g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
+        }
+        sort_w_.build(wsorts, stream2_);
+        NKR_HIP(hipEventRecord(ev_wsched_, stream2_));
+        batch_g2_.run(g2jobs, stream2_);
+    }
 
     // device tables: SpMV jobs, then per size class the transform and quotient pointer lists
     std::map<uint32_t, std::vector<size_t>> classes;
@@ -147,10 +189,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     NKR_HIP(hipMemcpyAsync(tables_.get(), tst, tbytes, hipMemcpyHostToDevice, stream_));
     table_stage_.uploaded(stream_);
 
-    // One stream: every stage is already one launch over the whole batch and
-    // fills the device, so a second stream would only make the stages wait
-    // for each other's workgroup slots.
-    // A*z, B*z, C*z, then the quotient chain per size class
+    // Main stream: A*z, B*z, C*z, then the quotient chain per size class
     spmv3_batch(reinterpret_cast<const SpmvJob *>(tables_.get()), (uint32_t)K, max_rows, max_long, stream_);
     for (const ClassTables &ct : ctabs) {
         fr_t *const *all3 = reinterpret_cast<fr_t *const *>(tables_.get() + ct.all3);
@@ -159,18 +198,17 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         ct.domain->dit_batch(all3, 3 * ct.count, false, ct.domain->coset_rev_scaled(), nullptr, stream_);
         ct.domain->quotient_batch(quot, ct.count, stream_);
     }
-    // digit schedules of every witness and every H, then one G1 batch over
-    // all four G1 MSMs of every proof and one G2 batch
+    // digit schedules of every H, then one G1 batch over all four G1 MSMs of
+    // every proof (the witness schedules from the side stream)
     std::vector<MsmJob<fq_t>> g1jobs;
-    std::vector<MsmJob<fq2_t>> g2jobs;
-    std::vector<MsmScheduleBatch::Item> sorts;
+    std::vector<MsmScheduleBatch::Item> hsorts;
     for (size_t i = 0; i < K; ++i) {
         Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
-        sorts.push_back({&sl.wsched, sl.z.get(), batch_plan(g.wplan_)});
-        sorts.push_back({&sl.hsched, sl.pa.get(), batch_plan(g.hplan_)});
+        hsorts.push_back({&sl.hsched, sl.pa.get(), batch_plan(g.hplan_)});
     }
-    sort_batch_.build(sorts, stream_);
+    sort_h_.build(hsorts, stream_);
+    NKR_HIP(hipStreamWaitEvent(stream_, ev_wsched_, 0));
     for (size_t i = 0; i < K; ++i) {
         Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
@@ -179,10 +217,8 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         g1jobs.push_back({&g.b1_, &sl.wsched, sl.planes1.get() + 1 * cw});
         g1jobs.push_back({&g.l_, &sl.wsched, sl.planes1.get() + 2 * cw});
         g1jobs.push_back({&g.h_, &sl.hsched, sl.planes1.get() + 3 * cw});
-        g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
     }
     batch_g1_.run(g1jobs, stream_);
-    batch_g2_.run(g2jobs, stream_);
 
     // bit planes of every proof to the host
     uint8_t *ph = planes_host_.reuse(pbytes);
@@ -195,9 +231,11 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         const size_t b2 = g.wplan_.c * sizeof(Xyzz<fq2_t>);
         poff[i] = off;
         NKR_HIP(hipMemcpyAsync(ph + off, sl.planes1.get(), b1, hipMemcpyDeviceToHost, stream_));
-        NKR_HIP(hipMemcpyAsync(ph + off + b1, sl.planes2.get(), b2, hipMemcpyDeviceToHost, stream_));
+        NKR_HIP(hipMemcpyAsync(ph + off + b1, sl.planes2.get(), b2, hipMemcpyDeviceToHost, stream2_));
         off += b1 + b2;
     }
+    NKR_HIP(hipEventRecord(ev_g2_, stream2_));
+    NKR_HIP(hipStreamWaitEvent(stream_, ev_g2_, 0));
 
     // r,s-only blinding terms while the GPU works, then the host tails
     std::vector<Groth16Prover::BlindPre> pre(K);
