@@ -104,4 +104,156 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
     return r;
 }
 
+// ---------------------------------------------------------------- G2 (Fq2)
+// Fq2 = Fq[u]/(u^2 + 1) over the 29-bit form.  fp29_mul needs a*b < 2^261 p
+// = 169.28 p^2; a Karatsuba product multiplies the sums of the components,
+// so the bounds are tighter than for G1 and some values are reduced:
+//   fq2_29_mul(a, b), components below A p and B p with A*B <= 40:
+//     components below 5p
+//   fq2_29_sqr(a), components below 5p: c0 below 1.6p, c1 below 2.3p
+//   accumulator  X, Y below 2p; ZZ, ZZZ below 5p; infinity is ZZ = 0 exactly
+//   base         x2, y2 below 8p (fp29_from_mont, then two subtractions)
+// madd-2008-s, the sequence of xyzz29_add_affine:
+//   U2 = x2 ZZ, S2 = y2 ZZZ (8p x 5p)                       below 5p
+//   P = U2 + 2p - X, R = S2 + 2p - Y, each reduced by 4p    below 4p
+//   PP = P^2, RR = R^2                                      below 2.3p
+//   PPP = P PP, Q = X PP, ZZ3 = ZZ PP                       below 5p
+//   X3 = (RR + 5p - PPP) + 10p - 2Q, reduced                below 2p
+//   Y3 = R (Q + 2p - X3) + 5p - Y PPP, reduced              below 2p
+//   ZZZ3 = ZZZ PPP                                          below 5p
+template <class P>
+struct Fq2_29 {
+    Fp29<P> c0, c1;
+    NKR_HD bool is_exact_zero() const { return c0.is_exact_zero() && c1.is_exact_zero(); }
+};
+
+template <class P>
+NKR_HD Fq2_29<P> fq2_29_from(const fq2_t &a)
+{
+    return {fp29_reduce_to_8p(fp29_from_mont(a.c0)), fp29_reduce_to_8p(fp29_from_mont(a.c1))};
+}
+
+template <class P>
+NKR_HD fq2_t fq2_29_to(const Fq2_29<P> &a)
+{
+    return {fp29_to_mont(a.c0), fp29_to_mont(a.c1)};
+}
+
+template <class P>
+NKR_HD Fq2_29<P> fq2_29_mul(const Fq2_29<P> &a, const Fq2_29<P> &b)
+{
+    const Fp29<P> v0 = fp29_mul(a.c0, b.c0);
+    const Fp29<P> v1 = fp29_mul(a.c1, b.c1);
+    const Fp29<P> v2 = fp29_mul(fp29_add(a.c0, a.c1), fp29_add(b.c0, b.c1));
+    return {fp29_sub<P, 2>(v0, v1), fp29_sub<P, 3>(v2, fp29_add(v0, v1))};
+}
+
+template <class P>
+NKR_HD Fq2_29<P> fq2_29_sqr(const Fq2_29<P> &a)
+{
+    const Fp29<P> m = fp29_mul(a.c0, a.c1);
+    return {fp29_mul(fp29_add(a.c0, a.c1), fp29_sub<P, 5>(a.c0, a.c1)), fp29_add(m, m)};
+}
+
+template <class P, uint32_t K>
+NKR_HD Fq2_29<P> fq2_29_sub(const Fq2_29<P> &a, const Fq2_29<P> &b)
+{
+    return {fp29_sub<P, K>(a.c0, b.c0), fp29_sub<P, K>(a.c1, b.c1)};
+}
+
+template <class P, uint32_t K>
+NKR_HD Fq2_29<P> fq2_29_sub_if_ge(const Fq2_29<P> &a)
+{
+    return {fp29_sub_if_ge<P, K>(a.c0), fp29_sub_if_ge<P, K>(a.c1)};
+}
+
+// a below 32p -> below 2p, component-wise
+template <class P>
+NKR_HD Fq2_29<P> fq2_29_reduce_to_2p(const Fq2_29<P> &a)
+{
+    return {fp29_reduce_to_2p(a.c0), fp29_reduce_to_2p(a.c1)};
+}
+
+template <class P>
+NKR_HD bool fq2_29_is_zero_below_4p(const Fq2_29<P> &a)
+{
+    if (!(fp29_may_be_zero<P, 4>(a.c0) && fp29_may_be_zero<P, 4>(a.c1))) return false;
+    return fp29_to_mont(a.c0).is_zero() && fp29_to_mont(a.c1).is_zero();
+}
+
+template <class P>
+struct Xyzz2_29 {
+    Fq2_29<P> X, Y, ZZ, ZZZ;
+    NKR_HD static Xyzz2_29 infinity()
+    {
+        Xyzz2_29 r;
+        r.X.c0 = r.X.c1 = r.Y.c0 = r.Y.c1 = Fp29<P>::zero();
+        r.ZZ.c0 = r.ZZ.c1 = r.ZZZ.c0 = r.ZZZ.c1 = Fp29<P>::zero();
+        return r;
+    }
+    NKR_HD bool is_inf() const { return ZZ.is_exact_zero(); }
+};
+
+// Canonical XYZZ over Fq2 -> the accumulator's form.
+template <class P>
+NKR_HD Xyzz2_29<P> xyzz2_29_from(const Xyzz<fq2_t> &p)
+{
+    if (p.ZZ.is_zero()) return Xyzz2_29<P>::infinity();
+    Xyzz2_29<P> r;
+    r.X = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.X.c0), fp29_from_mont(p.X.c1)});
+    r.Y = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.Y.c0), fp29_from_mont(p.Y.c1)});
+    r.ZZ = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.ZZ.c0), fp29_from_mont(p.ZZ.c1)});
+    r.ZZZ = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.ZZZ.c0), fp29_from_mont(p.ZZZ.c1)});
+    return r;
+}
+
+template <class P>
+NKR_HD Xyzz<fq2_t> xyzz2_29_to(const Xyzz2_29<P> &p)
+{
+    if (p.is_inf()) return Xyzz<fq2_t>::infinity();
+    Xyzz<fq2_t> r;
+    r.X = fq2_29_to(p.X);
+    r.Y = fq2_29_to(p.Y);
+    r.ZZ = fq2_29_to(p.ZZ);
+    r.ZZZ = fq2_29_to(p.ZZZ);
+    return r;
+}
+
+// acc + (x2, y2) over G2, complete.  x2, y2: the affine base from
+// fq2_29_from (below 8p).
+template <class P>
+NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &x2, const Fq2_29<P> &y2)
+{
+    if (acc.is_inf()) {
+        Xyzz2_29<P> r;
+        r.X = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(x2));
+        r.Y = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(y2));
+        r.ZZ.c0 = r.ZZZ.c0 = Fp29<P>::one();
+        r.ZZ.c1 = r.ZZZ.c1 = Fp29<P>::zero();
+        return r;
+    }
+    const Fq2_29<P> Pd = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(fq2_29_mul(x2, acc.ZZ), acc.X));
+    const Fq2_29<P> R = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(fq2_29_mul(y2, acc.ZZZ), acc.Y));
+    if (fq2_29_is_zero_below_4p(Pd)) {
+        // the base is +-(the accumulator): the canonical formulas decide
+        if (fq2_29_is_zero_below_4p(R))
+            return xyzz2_29_from<P>(xyzz_dbl_affine(fq2_29_to(x2), fq2_29_to(y2)));
+        return Xyzz2_29<P>::infinity();
+    }
+    const Fq2_29<P> PP = fq2_29_sqr(Pd);
+    const Fq2_29<P> PPP = fq2_29_mul(Pd, PP);
+    const Fq2_29<P> RR = fq2_29_sqr(R);
+    const Fq2_29<P> Q = fq2_29_mul(acc.X, PP);
+    Xyzz2_29<P> r;
+    r.ZZ = fq2_29_mul(acc.ZZ, PP);
+    const Fq2_29<P> Q2 = {fp29_add(Q.c0, Q.c0), fp29_add(Q.c1, Q.c1)};
+    r.X = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(fq2_29_sub_if_ge<P, 8>(
+        fq2_29_sub_if_ge<P, 16>(fq2_29_sub<P, 10>(fq2_29_sub<P, 5>(RR, PPP), Q2)))));
+    const Fq2_29<P> t1 = fq2_29_mul(R, fq2_29_sub<P, 2>(Q, r.X));
+    const Fq2_29<P> t2 = fq2_29_mul(acc.Y, PPP);
+    r.ZZZ = fq2_29_mul(acc.ZZZ, PPP);
+    r.Y = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(fq2_29_sub_if_ge<P, 8>(fq2_29_sub<P, 5>(t1, t2))));
+    return r;
+}
+
 }  // namespace nkr

@@ -432,17 +432,19 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
         const uint32_t *__restrict__ sorted = jb.sorted;
         Xyzz<F> acc = Xyzz<F>::infinity();
         if constexpr (std::is_same<F, fq2_t>::value) {
-            // G2: no prefetch; the second base's 32 registers would push the
-            // addition's working set past the register file of two waves.
+            // G2: the accumulator in the reduced-radix form as for G1
+            // (curve29.h, Fq2 bounds); no prefetch.
+            Xyzz2_29<FqParams> acc29 = Xyzz2_29<FqParams>::infinity();
 #pragma unroll 1
             for (uint32_t e = beg; e < end; ++e) {
                 const uint32_t v = sorted[e];
                 const Affine<F> P = bases[v & kEntryMask];
                 if (!affine_is_inf(P)) {
                     const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
-                    acc = xyzz_add_affine(acc, P.x, y);
+                    acc29 = xyzz2_29_add_affine(acc29, fq2_29_from<FqParams>(P.x), fq2_29_from<FqParams>(y));
                 }
             }
+            acc = xyzz2_29_to(acc29);
         } else if (beg < end) {
             // G1: the accumulator stays in the reduced-radix form
             // (curve29.h: one v_mad_u64_u32 per product term, no carries)

@@ -9,7 +9,11 @@
 //     canonical ones;
 //   * mixed addition: chains of xyzz29_add_affine over random bases, repeated
 //     bases (doubling), negated ones (infinity) and infinity restarts give,
-//     after every step, the canonical coordinates of xyzz_add_affine.
+//     after every step, the canonical coordinates of xyzz_add_affine;
+//   * the same over Fq2 (G2): Karatsuba products and squares at their input
+//     bounds, and chains of xyzz2_29_add_affine against xyzz_add_affine.
+//     The formulas are field identities, so random coordinates (not on the
+//     curve) exercise them as well as curve points do.
 // Prints "ok <checks>" or the first mismatches; exit status 1 on a mismatch.
 #include <cstdio>
 #include <vector>
@@ -133,11 +137,67 @@ static void check_madd(int chains, int len)
     }
 }
 
+static fq2_t rand_fq2() { return {rand_fp<FqParams>(), rand_fp<FqParams>()}; }
+
+// a + K*p per component (values below (K + 2) p)
+template <uint32_t K>
+static Fq2_29<FqParams> widen2(const fq2_t &a)
+{
+    return {widen<FqParams, K>(a.c0), widen<FqParams, K>(a.c1)};
+}
+
+static void check_fq2(int n)
+{
+    for (int i = 0; i < n; ++i) {
+        const fq2_t a = rand_fq2(), b = rand_fq2();
+        const fq2_t want = fq2_mul(a, b);
+        expect(fq2_29_to(fq2_29_mul(widen2<6>(a), widen2<3>(b))) == want, "fq2 mul 8p x 5p", i);
+        expect(fq2_29_to(fq2_29_mul(widen2<3>(a), widen2<6>(b))) == want, "fq2 mul 5p x 8p", i);
+        expect(fq2_29_to(fq2_29_sqr(widen2<3>(a))) == fq2_sqr(a), "fq2 sqr 5p", i);
+        const Fq2_29<FqParams> m = fq2_29_mul(widen2<6>(a), widen2<3>(b));
+        bool limbs = true;
+        for (int k = 0; k < 9; ++k) limbs = limbs && (m.c0.l[k] >> 29) == 0 && (m.c1.l[k] >> 29) == 0;
+        expect(limbs, "fq2 product limbs", i);
+        expect(fq2_29_to(fq2_29_from<FqParams>(a)) == a, "fq2 round trip", i);
+    }
+}
+
+static void check_madd_g2(int chains, int len)
+{
+    std::vector<Affine<fq2_t>> pool;
+    for (int i = 0; i < 24; ++i) pool.push_back({rand_fq2(), rand_fq2()});
+    for (int c = 0; c < chains; ++c) {
+        Xyzz<fq2_t> a32 = Xyzz<fq2_t>::infinity();
+        Xyzz2_29<FqParams> a29 = Xyzz2_29<FqParams>::infinity();
+        for (int e = 0; e < len; ++e) {
+            const uint32_t r = rnd();
+            Affine<fq2_t> b = pool[r % pool.size()];
+            if ((r >> 8) % 5 == 0 && !a32.is_inf()) {
+                b = xyzz_to_affine(a32);
+                ++g_doublings;
+            }
+            if ((r >> 12) % 7 == 0) b.y = fq2_neg(b.y);
+            if ((r >> 16) % 11 == 0 && !a32.is_inf()) {
+                b = xyzz_to_affine(a32);
+                b.y = fq2_neg(b.y);
+                ++g_cancels;
+            }
+            a32 = xyzz_add_affine(a32, b.x, b.y);
+            a29 = xyzz2_29_add_affine(a29, fq2_29_from<FqParams>(b.x), fq2_29_from<FqParams>(b.y));
+            const Xyzz<fq2_t> got = xyzz2_29_to(a29);
+            expect(got.X == a32.X && got.Y == a32.Y && got.ZZ == a32.ZZ && got.ZZZ == a32.ZZZ, "g2 madd chain",
+                   c * len + e);
+        }
+    }
+}
+
 int main()
 {
     check_field<FqParams>(20000);
     check_field<FrParams>(20000);
     check_madd(200, 80);
+    check_fq2(5000);
+    check_madd_g2(60, 60);
     if (g_bad) {
         std::printf("FAILED %d of %ld checks\n", g_bad, g_checks);
         return 1;
