@@ -158,15 +158,25 @@ int nkr_g16_batch_destroy(nkr_g16_batch *batch);
  * scripts/bench/runner.py:260-355).  Contexts stay resident (keys and NTT
  * tables in HBM); submit() queues a proof and returns a ticket; `workers`
  * threads each own a batch (own streams), and a free worker takes every
- * pending request that fits the work-memory budget, up to max_batch, so
- * requests arriving while other batches are on the GPU are admitted as
- * workers and memory free up and run alongside them.  Each request is
- * charged nkr_g16_server_request_bytes of its context; a request larger than
- * the whole budget is rejected by submit.  wait() blocks until the ticket is
- * done and gives its submit-to-completion latency.  Results are exactly
- * nkr_g16_prove_blind's.  The caller keeps witness, r, s and the outputs
- * alive until wait() returns; submit and wait may be called from any thread.
- * stats[4] = batches run, proofs done, largest batch, peak admitted bytes. */
+ * pending request whose charges fit its share of the work-memory budget, up
+ * to max_batch, so requests arriving while other batches are on the GPU are
+ * admitted as workers and memory free up and run alongside them.  The budget
+ * is clamped at creation to the device's free memory minus a reserve (the
+ * larger of 1 GiB and 1/32 of the device) and split equally between the
+ * workers; each request is charged nkr_g16_server_request_bytes of its
+ * context, and a request larger than one worker's share is rejected by
+ * submit.  A worker releases its grow-only work buffers whenever keeping them
+ * would exceed its share, so the work memory held by all workers stays within
+ * the budget.  A request that cannot be proved (its context lacks the
+ * blinding elements) fails alone: its wait() reports the error and the rest
+ * of its batch completes; a batch that fails as a whole is retried request by
+ * request.  wait() blocks until the ticket is done and gives its
+ * submit-to-completion latency.  Results are exactly nkr_g16_prove_blind's.
+ * The caller keeps witness, r, s and the outputs alive until wait() returns;
+ * submit and wait may be called from any thread.
+ * stats[4] = batches run, proofs done, largest batch, peak admitted bytes.
+ * memory[4] = effective budget, one worker's share, work bytes the workers
+ * hold now, most work bytes held at once. */
 typedef struct nkr_g16_server nkr_g16_server;
 int nkr_g16_server_create(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes,
                           nkr_g16_server **out);
@@ -175,6 +185,7 @@ int nkr_g16_server_submit(nkr_g16_server *server, nkr_g16_ctx *ctx, const void *
 int nkr_g16_server_wait(nkr_g16_server *server, uint64_t ticket, double *latency_ms);
 int nkr_g16_server_request_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes);
 int nkr_g16_server_stats(const nkr_g16_server *server, uint64_t *stats);
+int nkr_g16_server_memory(const nkr_g16_server *server, uint64_t *memory);
 int nkr_g16_server_destroy(nkr_g16_server *server);
 
 /* Stage timing with HIP events on the context's stream.  While enabled,

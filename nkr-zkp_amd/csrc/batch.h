@@ -52,6 +52,11 @@ public:
 
     void prove(const std::vector<BatchItem> &items);
     int device() const { return device_; }
+    // Device bytes the per-proof slots and the batch's work buffers hold
+    // (they grow to the largest batch run), and their release: the next
+    // prove reallocates what it needs.
+    uint64_t device_bytes() const;
+    void trim();
 
 private:
     void release() noexcept;  // streams and events (destructor, failed constructor)

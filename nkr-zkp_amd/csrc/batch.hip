@@ -86,6 +86,29 @@ void BatchProver::release() noexcept
         }
 }
 
+uint64_t BatchProver::device_bytes() const
+{
+    uint64_t b = tables_.bytes() + sort_w_.device_bytes() + sort_h_.device_bytes() + batch_g1_.device_bytes() +
+                 batch_g2_.device_bytes();
+    for (const auto &sl : slots_)
+        b += sl->z.bytes() + sl->pa.bytes() + sl->pb.bytes() + sl->pc.bytes() + sl->wsched.device_bytes() +
+             sl->hsched.device_bytes() + sl->planes1.bytes() + sl->planes2.bytes();
+    return b;
+}
+
+void BatchProver::trim()
+{
+    NKR_HIP(hipSetDevice(device_));
+    NKR_HIP(hipStreamSynchronize(stream_));
+    NKR_HIP(hipStreamSynchronize(stream2_));
+    slots_.clear();
+    tables_.release();
+    sort_w_.release_device();
+    sort_h_.release_device();
+    batch_g1_.release_device();
+    batch_g2_.release_device();
+}
+
 void BatchProver::prove(const std::vector<BatchItem> &items)
 {
     const size_t K = items.size();

@@ -351,6 +351,18 @@ int nkr_g16_server_stats(const nkr_g16_server *server, uint64_t *stats)
     });
 }
 
+int nkr_g16_server_memory(const nkr_g16_server *server, uint64_t *memory)
+{
+    return guarded([&] {
+        NKR_REQUIRE(server && memory, "null argument");
+        const Server::Memory m = server->server->memory();
+        memory[0] = m.budget;
+        memory[1] = m.share;
+        memory[2] = m.held;
+        memory[3] = m.peak_held;
+    });
+}
+
 int nkr_g16_server_destroy(nkr_g16_server *server)
 {
     return guarded([&] { delete server; });

@@ -121,6 +121,22 @@ public:
         return totals_host() + 4;
     }
 
+    // Device bytes held, and their release (the next prepare reallocates).
+    uint64_t device_bytes() const
+    {
+        return sorted_.bytes() + count_.bytes() + bucket_start_.bytes() + piece_start_.bytes() +
+               block_count_.bytes() + alt_piece_start_.bytes();
+    }
+    void release_device()
+    {
+        sorted_.release();
+        count_.release();
+        bucket_start_.release();
+        piece_start_.release();
+        block_count_.release();
+        alt_piece_start_.release();
+    }
+
     MsmPlan plan_;
     uint32_t blocks_ = 0;
     uint32_t target_log_ = kMsmTargetPiecesLog;  // as of the last prepare
@@ -148,6 +164,12 @@ public:
         MsmPlan plan;
     };
     void build(const std::vector<Item> &items, hipStream_t stream);
+    uint64_t device_bytes() const { return table_.bytes() + totals_dev_.bytes(); }
+    void release_device()
+    {
+        table_.release();
+        totals_dev_.release();
+    }
 
 private:
     PinnedStaging<uint8_t> stage_;
@@ -172,6 +194,31 @@ template <class F>
 class MsmBatch {
 public:
     void run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer = nullptr);
+    // Device bytes held by the batch's work buffers, and their release (the
+    // next run reallocates what it needs).
+    uint64_t device_bytes() const
+    {
+        uint64_t b = complete_base_.bytes() + class_count_.bytes() + class_list_.bytes() + jobs_dev_.bytes() +
+                     thread_base_dev_.bytes() + pool_.bytes() + sums_.bytes() + reg_start_.bytes() +
+                     reg_count_.bytes() + fold_start_.bytes();
+        return b + tree_[0].bytes() + tree_[1].bytes();
+    }
+    void release_device()
+    {
+        complete_base_.release();
+        class_count_.release();
+        class_list_.release();
+        jobs_dev_.release();
+        thread_base_dev_.release();
+        pool_.release();
+        sums_.release();
+        tree_[0].release();
+        tree_[1].release();
+        reg_start_.release();
+        reg_count_.release();
+        fold_start_.release();
+        key_.clear();  // the layout cache refers to the released buffers
+    }
 
 private:
     void layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream);

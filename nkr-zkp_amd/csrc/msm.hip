@@ -781,7 +781,11 @@ template <class F>
 uint64_t msm_job_bytes(const MsmPlan &p)
 {
     const uint64_t total = msm_partial_capacity(p, p.max_pieces(msm_target_pieces_log()), nullptr, nullptr);
-    return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12) + p.c * sizeof(Xyzz<F>);
+    // partial sums; per bucket: sums and two tree levels, region start,
+    // count and fold start, the completion class lists; the bit planes; and
+    // the job's fixed records (job table, class counters, piece offsets)
+    return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12 + 4 * kCompleteClasses) +
+           p.c * sizeof(Xyzz<F>) + 1024;
 }
 
 template <class F>

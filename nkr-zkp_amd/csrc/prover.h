@@ -51,6 +51,8 @@ public:
 
     uint64_t device_bytes() const;
     int device() const { return device_; }
+    // prepared with alpha/beta/delta, which blinding needs
+    bool can_blind() const { return have_blind_; }
     hipStream_t stream() const { return stream_; }
     uint32_t rows() const { return rows_; }
     uint32_t cols() const { return cols_; }

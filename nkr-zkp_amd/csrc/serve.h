@@ -7,10 +7,24 @@
 // completes them; meanwhile the other workers' batches are on the GPU on
 // their own streams, so requests that arrive while the GPU is busy are
 // admitted as soon as a worker and memory free up and their work overlaps
-// the batches already running.  Admission charges each request the device
+// the batches already running.
+//
+// Device memory: the work budget is clamped at creation to the device's
+// free memory minus a reserve (the larger of 1 GiB and 1/32 of the device),
+// and each worker gets an equal share of it.  A request is charged the device
 // bytes its batch slot needs (witness, three domain polynomials, the two
-// digit schedules and MSM partial sums); a request that alone exceeds the
-// budget is rejected at submission.
+// digit schedules, MSM partial sums and per-bucket arrays); a worker admits
+// requests while their charges fit its share, and a request larger than a
+// share is rejected at submission.  A worker's batch prover keeps its
+// buffers between batches (they only grow), so before a batch whose charges
+// would take it over its share, and after any batch that left it above its
+// share, the worker releases them: the work memory all workers hold stays
+// within the budget.
+//
+// Failures: requests that cannot be proved (a context prepared without the
+// blinding elements, a context on another device) fail alone, at
+// admission; if a whole batch fails, its requests are retried one by one so
+// that only the ones that fail again report an error.
 #pragma once
 
 #include <chrono>
@@ -51,6 +65,13 @@ public:
     };
     Stats stats() const;
 
+    // Work memory: the effective budget, one worker's share, the bytes the
+    // workers' batch provers hold now, and the most they held at once.
+    struct Memory {
+        uint64_t budget = 0, share = 0, held = 0, peak_held = 0;
+    };
+    Memory memory() const;
+
 private:
     struct Request {
         uint64_t ticket;
@@ -64,10 +85,15 @@ private:
         std::string error;
     };
     void worker_loop(uint32_t index);
+    // Proves a batch; on failure retries its requests one by one.  Returns
+    // one error message per request (empty on success).
+    std::vector<std::string> prove_batch(BatchProver &prover, const std::vector<Request> &batch);
 
     int device_;
     uint32_t max_batch_;
-    uint64_t budget_;
+    uint64_t budget_ = 0, share_ = 0;
+    std::vector<uint64_t> held_;  // per worker: bytes its batch prover holds
+    uint64_t peak_held_ = 0;
     mutable std::mutex mu_;
     std::condition_variable cv_work_, cv_done_;
     std::deque<Request> pending_;

@@ -15,10 +15,21 @@ uint64_t Server::request_bytes(const Groth16Prover &g)
 }
 
 Server::Server(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes)
-    : device_(device), max_batch_(max_batch), budget_(work_budget_bytes)
+    : device_(device), max_batch_(max_batch)
 {
     NKR_REQUIRE(workers >= 1 && workers <= 8, "a server runs 1 .. 8 workers");
     NKR_REQUIRE(max_batch >= 1, "max_batch must be at least 1");
+    NKR_REQUIRE(work_budget_bytes > 0, "the work budget must be positive");
+    NKR_HIP(hipSetDevice(device_));
+    size_t free_b = 0, total_b = 0;
+    NKR_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t reserve = std::max<uint64_t>(1ull << 30, total_b / 32);
+    const uint64_t avail = free_b > reserve ? free_b - reserve : 0;
+    budget_ = std::min<uint64_t>(work_budget_bytes, avail);
+    NKR_REQUIRE(budget_ > 0, "no device memory free for proving work (" + std::to_string(free_b) +
+                                 " bytes free, " + std::to_string(reserve) + " kept in reserve)");
+    share_ = budget_ / workers;
+    held_.assign(workers, 0);
     for (uint32_t i = 0; i < workers; ++i) provers_.push_back(std::make_unique<BatchProver>(device));
     for (uint32_t i = 0; i < workers; ++i) threads_.emplace_back([this, i] { worker_loop(i); });
 }
@@ -39,9 +50,10 @@ uint64_t Server::submit(Groth16Prover *ctx, const void *witness, const fr_t &r, 
     NKR_REQUIRE(ctx && witness && raw_out && proof_out, "null request member");
     NKR_REQUIRE(ctx->device() == device_, "context prepared on another device");
     const uint64_t bytes = request_bytes(*ctx);
-    NKR_REQUIRE(bytes <= budget_, "request needs " + std::to_string(bytes) +
-                                      " bytes of work memory, more than the server's budget of " +
-                                      std::to_string(budget_));
+    NKR_REQUIRE(bytes <= share_, "request needs " + std::to_string(bytes) +
+                                     " bytes of work memory, more than a worker's share of " +
+                                     std::to_string(share_) + " (a budget of " + std::to_string(budget_) +
+                                     " over " + std::to_string(provers_.size()) + " workers)");
     std::lock_guard<std::mutex> lk(mu_);
     NKR_REQUIRE(!stop_, "server is shutting down");
     Request q;
@@ -73,6 +85,55 @@ Server::Stats Server::stats() const
     return stats_;
 }
 
+Server::Memory Server::memory() const
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    Memory m;
+    m.budget = budget_;
+    m.share = share_;
+    for (uint64_t h : held_) m.held += h;
+    m.peak_held = peak_held_;
+    return m;
+}
+
+std::vector<std::string> Server::prove_batch(BatchProver &prover, const std::vector<Request> &batch)
+{
+    std::vector<std::string> errors(batch.size());
+    // requests that cannot be proved fail alone, before the batch runs
+    std::vector<BatchItem> items;
+    std::vector<size_t> at;
+    for (size_t i = 0; i < batch.size(); ++i) {
+        const BatchItem &it = batch[i].item;
+        if (!it.ctx->can_blind())
+            errors[i] = "the context was prepared without alpha/beta/delta, which blinding needs";
+        else if (it.ctx->device() != device_)
+            errors[i] = "context prepared on another device";
+        else {
+            items.push_back(it);
+            at.push_back(i);
+        }
+    }
+    if (items.empty()) return errors;
+    try {
+        prover.prove(items);
+        return errors;
+    } catch (const std::exception &e) {
+        if (items.size() == 1) {
+            errors[at[0]] = e.what();
+            return errors;
+        }
+    }
+    // the batch failed as a whole: one by one, so that only failing requests fail
+    for (size_t k = 0; k < items.size(); ++k) {
+        try {
+            prover.prove({items[k]});
+        } catch (const std::exception &e) {
+            errors[at[k]] = e.what();
+        }
+    }
+    return errors;
+}
+
 void Server::worker_loop(uint32_t index)
 {
     BatchProver &prover = *provers_[index];
@@ -81,12 +142,11 @@ void Server::worker_loop(uint32_t index)
         uint64_t bytes = 0;
         {
             std::unique_lock<std::mutex> lk(mu_);
-            // admit pending requests in arrival order while memory allows
-            cv_work_.wait(lk, [&] {
-                return pending_.empty() ? stop_ : in_use_ + pending_.front().bytes <= budget_;
-            });
+            // admit pending requests in arrival order while their charges fit
+            // this worker's share (every request fits a share: submit checks)
+            cv_work_.wait(lk, [&] { return !pending_.empty() || stop_; });
             if (stop_ && pending_.empty()) return;
-            while (!pending_.empty() && batch.size() < max_batch_ && in_use_ + bytes + pending_.front().bytes <= budget_) {
+            while (!pending_.empty() && batch.size() < max_batch_ && bytes + pending_.front().bytes <= share_) {
                 bytes += pending_.front().bytes;
                 batch.push_back(pending_.front());
                 pending_.pop_front();
@@ -98,23 +158,39 @@ void Server::worker_loop(uint32_t index)
             stats_.proofs += batch.size();
             stats_.max_batch_seen = std::max<uint64_t>(stats_.max_batch_seen, batch.size());
         }
-        std::vector<BatchItem> items;
-        for (const Request &q : batch) items.push_back(q.item);
-        std::string error;
+        // The prover's buffers from earlier batches stay unless this batch's
+        // charges would take it over the share.
+        std::vector<std::string> errors(batch.size());
+        uint64_t held = 0;
         try {
-            prover.prove(items);
+            if (prover.device_bytes() + bytes > share_) prover.trim();
+            errors = prove_batch(prover, batch);
+            held = prover.device_bytes();
         } catch (const std::exception &e) {
-            error = e.what();
+            for (std::string &err : errors) err = e.what();
+        }
+        uint64_t kept = held;
+        if (held > share_) {
+            try {
+                prover.trim();
+                kept = prover.device_bytes();
+            } catch (...) {
+            }
         }
         const auto t_end = std::chrono::steady_clock::now();
         {
             std::lock_guard<std::mutex> lk(mu_);
             in_use_ -= bytes;
-            for (const Request &q : batch) {
-                Done &d = done_[q.ticket];
+            held_[index] = held;
+            uint64_t sum = 0;
+            for (uint64_t h : held_) sum += h;
+            peak_held_ = std::max(peak_held_, sum);
+            held_[index] = kept;
+            for (size_t i = 0; i < batch.size(); ++i) {
+                Done &d = done_[batch[i].ticket];
                 d.finished = true;
-                d.error = error;
-                d.latency_ms = std::chrono::duration<double, std::milli>(t_end - q.t_submit).count();
+                d.error = errors[i];
+                d.latency_ms = std::chrono::duration<double, std::milli>(t_end - batch[i].t_submit).count();
             }
         }
         cv_done_.notify_all();

@@ -136,6 +136,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_server_wait": [vp, u64, ctypes.POINTER(ctypes.c_double)],
             "nkr_g16_server_request_bytes": [vp, ctypes.POINTER(u64)],
             "nkr_g16_server_stats": [vp, ctypes.POINTER(u64)],
+            "nkr_g16_server_memory": [vp, ctypes.POINTER(u64)],
             "nkr_g16_server_destroy": [vp],
             "nkr_bn254_pairing": [vp, vp, vp],
             "nkr_bn254_pairing_product": [vp, vp, u64, vp],
@@ -435,6 +436,11 @@ class Server:
         _check(lib().nkr_g16_server_stats(self.h, st))
         return {"batches": st[0], "proofs": st[1], "max_batch": st[2], "peak_work_bytes": st[3]}
 
+    def memory(self) -> dict:
+        m = (ctypes.c_uint64 * 4)()
+        _check(lib().nkr_g16_server_memory(self.h, m))
+        return {"budget": m[0], "share": m[1], "held": m[2], "peak_held": m[3]}
+
     @staticmethod
     def request_bytes(pv) -> int:
         b = ctypes.c_uint64()
@@ -472,10 +478,16 @@ class Prover:
         self.h = h
 
     def _pk(self, pk) -> Pk:
+        """The proving key view; alpha/beta/delta may be absent (None): such a
+        context proves but cannot blind."""
         arrs = {}
         for k in ("a_query", "b_g1_query", "b_g2_query", "l_query", "h_query",
                   "alpha_g1", "beta_g1", "beta_g2", "delta_g1", "delta_g2"):
-            a = np.ascontiguousarray(np.asarray(getattr(pk, k), dtype=np.uint64))
+            v = getattr(pk, k, None)
+            if v is None and k.startswith(("alpha", "beta", "delta")):
+                arrs[k] = None
+                continue
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.uint64))
             self._keep.append(a)
             arrs[k] = a.ctypes.data
         return Pk(**arrs)

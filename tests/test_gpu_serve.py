@@ -1,8 +1,8 @@
 """Serving (nkr_g16_server_*): requests submitted while others are on the GPU
 are admitted and batched as workers and memory free up; every result equals
-the proof of the same request made alone, and verifies.  Memory admission
-rejects a request larger than the budget and still completes everything
-under a budget that fits one request at a time."""
+the proof of the same request made alone, and verifies.  The work memory the
+workers hold stays within the budget, the budget is clamped to the device's
+free memory, and a request that cannot be proved fails alone."""
 import threading
 import time
 
@@ -84,20 +84,92 @@ def test_concurrent_submitters(contexts):
 
 
 def test_memory_admission(contexts):
+    """Each of the workers gets an equal share of the budget: a request larger
+    than a share is rejected at submission; with a share that fits exactly one
+    request, every batch holds one request and everything still completes."""
     name, c, k, pv, wits = contexts[0]
     need = N.Server.request_bytes(pv)
     assert need > 0
-    small = N.Server(workers=2, max_batch=8, budget_bytes=need - 1)
-    with pytest.raises(N.NkrError, match="budget"):
+    small = N.Server(workers=2, max_batch=8, budget_bytes=2 * need - 2)
+    with pytest.raises(N.NkrError, match="share"):
         small.submit(pv, wits[0], *circuits.random_scalars(2, 1))
     small.close()
-    # room for exactly one request: everything still completes, one at a time
-    one = N.Server(workers=2, max_batch=8, budget_bytes=need)
+    one = N.Server(workers=2, max_batch=8, budget_bytes=2 * need)
     rs = circuits.random_scalars(8, 5)
     ts = [one.submit(pv, wits[i % 2], rs[2 * i], rs[2 * i + 1]) for i in range(4)]
     outs = [one.wait(t) for t in ts]
     st = one.stats()
+    mem = one.memory()
     one.close()
-    assert st["max_batch"] == 1 and st["peak_work_bytes"] == need
+    assert st["max_batch"] == 1 and need <= st["peak_work_bytes"] <= 2 * need
+    assert mem["share"] == need and mem["peak_held"] <= 2 * need
     for i, (raw, proof, _) in enumerate(outs):
         assert np.array_equal(proof, pv.prove_blind(wits[i % 2], rs[2 * i], rs[2 * i + 1])[1])
+
+
+def test_work_memory_stays_within_the_budget(contexts):
+    """Three workers with a budget that fits a little more than one of the
+    largest requests per worker: the work buffers the workers hold never add
+    up to more than the budget (the workers release their grow-only buffers
+    when keeping them would exceed their share), and every result still
+    equals the proof made alone."""
+    big = max(N.Server.request_bytes(pv) for (_, _, _, pv, _) in contexts)
+    budget = 3 * (big + big // 2)
+    reqs = requests(contexts, 30, 23)
+    alone = [pv.prove_blind(w, r, s)[1] for (_, _, _, pv, w, r, s) in reqs]
+    srv = N.Server(workers=3, max_batch=16, budget_bytes=budget)
+    try:
+        mem0 = srv.memory()
+        assert mem0["budget"] == budget and mem0["share"] == budget // 3
+        tickets = [srv.submit(q[3], q[4], q[5], q[6]) for q in reqs]
+        got = [srv.wait(t)[1] for t in tickets]
+        mem = srv.memory()
+        st = srv.stats()
+    finally:
+        srv.close()
+    for g, want in zip(got, alone):
+        assert np.array_equal(g, want)
+    assert 0 < mem["peak_held"] <= mem["budget"], mem
+    assert mem["held"] <= mem["budget"]
+    assert st["proofs"] == len(reqs)
+
+
+def test_budget_is_clamped_to_free_device_memory(contexts):
+    srv = N.Server(workers=2, max_batch=4, budget_bytes=1 << 60)
+    try:
+        mem = srv.memory()
+    finally:
+        srv.close()
+    assert 0 < mem["budget"] < 1 << 50 and mem["share"] == mem["budget"] // 2
+
+
+def test_request_without_blinding_keys_fails_alone(contexts):
+    """A context prepared without alpha/beta/delta cannot blind: its request
+    fails with that error while the requests batched with it complete."""
+    name, c, k, pv, wits = contexts[0]
+
+    class NoBlind:
+        pass
+
+    nb = NoBlind()
+    for attr in ("a_query", "b_g1_query", "b_g2_query", "l_query", "h_query"):
+        setattr(nb, attr, getattr(k, attr))
+    bad = N.Prover(c, nb)
+    reqs = requests(contexts, 8, 31)
+    alone = [pv_.prove_blind(w, r, s)[1] for (_, _, _, pv_, w, r, s) in reqs]
+    rs = circuits.random_scalars(2, 41)
+    srv = N.Server(workers=1, max_batch=16)
+    try:
+        tickets = [srv.submit(q[3], q[4], q[5], q[6]) for q in reqs[:4]]
+        t_bad = srv.submit(bad, wits[0], rs[0], rs[1])
+        tickets += [srv.submit(q[3], q[4], q[5], q[6]) for q in reqs[4:]]
+        got = [srv.wait(t)[1] for t in tickets]
+        with pytest.raises(N.NkrError, match="alpha/beta/delta"):
+            srv.wait(t_bad)
+        st = srv.stats()
+    finally:
+        srv.close()
+        bad.close()
+    for g, want in zip(got, alone):
+        assert np.array_equal(g, want)
+    assert st["proofs"] == len(reqs) + 1
