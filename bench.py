@@ -300,9 +300,19 @@ def main() -> int:
                     help="only time the standalone operators (NTT, G1/G2 MSM) at the operator-configuration "
                          "sizes and print them as one JSON line")
     ap.add_argument("--op-reps", type=int, default=10)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only run the batch leg (the 64-proof mix, plain and batched) and print it as one JSON "
+                         "line: the workload of the batch kernel trace")
     args = ap.parse_args()
     if args.pmc_workload:
         return pmc_workload(args.size, args.density)
+    if args.batch_only:
+        shapes, items = build_mix(args.batch_per_shape)
+        print(json.dumps({"batch": batch_leg(shapes, items, args.batch_per_shape, args.batch_reps),
+                          "data": "synthetic (seeded)"}), flush=True)
+        for pv in list({id(it[2]): it[2] for it in items}.values()):
+            pv.close()
+        return 0
     if args.operators:
         print(json.dumps({"operators": operators_leg(args.op_reps), "data": "synthetic (seeded)",
                           "inputs": "resident in HBM"}), flush=True)
