@@ -15,7 +15,15 @@
 // Two streams, as in the plain prover: the witness schedules and the G2 MSMs
 // need only the witnesses, so they run beside the SpMV, transforms, H
 // schedules and G1 MSMs; each side's latency-bound reduction levels then
-// overlap the other side's accumulation.
+// overlap the other side's accumulation.  (Moving the G1 MSMs over the
+// witnesses to the side stream as well, with the accumulations holding part
+// of the resident slots, measured no faster on MI355X: the batch is bound by
+// the accumulations' total work.)
+//   witnesses   host threads copy them into pinned memory in 1 MiB units
+//               while the calling thread issues each unit's host-to-device
+//               copy as soon as it is staged.
+//   bit planes  three batch-wide regions (G1 over the witness, H, G2), one
+//               device-to-host copy each.
 //   host tail   the r,s-only blinding terms while the GPU works, then the
 //               final sums and blinding on a pool of host threads.
 // Each proof's result is bit-identical to proving it alone.
@@ -63,19 +71,19 @@ private:
     struct Slot {
         DeviceBuffer<fr_t> z, pa, pb, pc;
         MsmSchedule wsched, hsched;
-        DeviceBuffer<Xyzz<fq_t>> planes1;
-        DeviceBuffer<Xyzz<fq2_t>> planes2;
     };
     int device_;
     hipStream_t stream_ = nullptr;   // SpMV, transforms, H schedules, G1 MSMs
     hipStream_t stream2_ = nullptr;  // witness schedules and G2 MSMs, beside them
-    hipEvent_t ev_z_ = nullptr, ev_wsched_ = nullptr, ev_g2_ = nullptr;
+    hipEvent_t ev_z_ = nullptr, ev_wsched_ = nullptr, ev_side_ = nullptr;
     std::vector<std::unique_ptr<Slot>> slots_;
     MsmScheduleBatch sort_w_, sort_h_;
     MsmBatch<fq_t> batch_g1_;
     MsmBatch<fq2_t> batch_g2_;
     PinnedStaging<uint8_t> witness_stage_, table_stage_;
     DeviceBuffer<uint8_t> tables_;
+    DeviceBuffer<Xyzz<fq_t>> planes_w_, planes_h_;  // bit planes: ar, bs1, zk of every proof; H of every proof
+    DeviceBuffer<Xyzz<fq2_t>> planes_2_;            // bs2 of every proof
     PinnedStaging<uint8_t> planes_host_;
 };
 

@@ -42,6 +42,9 @@ void parallel_for(size_t n, Fn fn)
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 constexpr uint32_t kBatchPiece = 32;
+// Witness staging unit (bytes): small enough that the first copies start
+// early, large enough that each copy moves at full PCIe rate.
+constexpr size_t kStageUnit = size_t(1) << 20;
 
 }  // namespace
 
@@ -62,7 +65,7 @@ BatchProver::BatchProver(int device) : device_(device)
     NKR_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
     NKR_HIP(hipEventCreateWithFlags(&ev_z_, hipEventDisableTiming));
     NKR_HIP(hipEventCreateWithFlags(&ev_wsched_, hipEventDisableTiming));
-    NKR_HIP(hipEventCreateWithFlags(&ev_g2_, hipEventDisableTiming));
+    NKR_HIP(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
     undo.armed = false;
 }
 
@@ -79,7 +82,7 @@ void BatchProver::release() noexcept
             (void)hipStreamDestroy(*s);
             *s = nullptr;
         }
-    for (hipEvent_t *e : {&ev_z_, &ev_wsched_, &ev_g2_})
+    for (hipEvent_t *e : {&ev_z_, &ev_wsched_, &ev_side_})
         if (*e) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -89,10 +92,11 @@ void BatchProver::release() noexcept
 uint64_t BatchProver::device_bytes() const
 {
     uint64_t b = tables_.bytes() + sort_w_.device_bytes() + sort_h_.device_bytes() + batch_g1_.device_bytes() +
-                 batch_g2_.device_bytes();
+                 batch_g2_.device_bytes() + planes_w_.bytes() + planes_h_.bytes() +
+                 planes_2_.bytes();
     for (const auto &sl : slots_)
         b += sl->z.bytes() + sl->pa.bytes() + sl->pb.bytes() + sl->pc.bytes() + sl->wsched.device_bytes() +
-             sl->hsched.device_bytes() + sl->planes1.bytes() + sl->planes2.bytes();
+             sl->hsched.device_bytes();
     return b;
 }
 
@@ -103,6 +107,9 @@ void BatchProver::trim()
     NKR_HIP(hipStreamSynchronize(stream2_));
     slots_.clear();
     tables_.release();
+    planes_w_.release();
+    planes_h_.release();
+    planes_2_.release();
     sort_w_.release_device();
     sort_h_.release_device();
     batch_g1_.release_device();
@@ -122,22 +129,42 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     NKR_HIP(hipSetDevice(device_));
     while (slots_.size() < K) slots_.push_back(std::make_unique<Slot>());
 
-    // witnesses -> pinned staging -> slot buffers
-    size_t wbytes = 0, pbytes = 0;
+    // witnesses -> pinned staging -> slot buffers; bit planes of every proof
+    // in three batch-wide regions (G1 over the witness, H, G2) so that each
+    // reaches the host in one copy
+    size_t wbytes = 0;
     uint32_t max_rows = 0, max_long = 0;
-    for (const BatchItem &it : items) {
-        const Groth16Prover &g = *it.ctx;
+    std::vector<size_t> pw_off(K + 1, 0), ph_off(K + 1, 0), p2_off(K + 1, 0);
+    for (size_t i = 0; i < K; ++i) {
+        const Groth16Prover &g = *items[i].ctx;
         for (int m = 0; m < 3; ++m) max_long = std::max(max_long, g.views_[m].n_long);
         wbytes += (size_t)g.cols_ * sizeof(fr_t);
-        pbytes += (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>) + g.wplan_.c * sizeof(Xyzz<fq2_t>);
         max_rows = std::max(max_rows, g.rows_);
+        pw_off[i + 1] = pw_off[i] + 3 * g.wplan_.c;
+        ph_off[i + 1] = ph_off[i] + g.hplan_.c;
+        p2_off[i + 1] = p2_off[i] + g.wplan_.c;
     }
+    planes_w_.ensure(pw_off[K]);
+    planes_h_.ensure(ph_off[K]);
+    planes_2_.ensure(p2_off[K]);
     uint8_t *wst = witness_stage_.reuse(wbytes);
-    std::vector<size_t> woff(K + 1, 0);
-    for (size_t i = 0; i < K; ++i) woff[i + 1] = woff[i] + (size_t)items[i].ctx->cols_ * sizeof(fr_t);
-    // host copies into the pinned staging buffer in parallel (host memory bandwidth)
-    parallel_for(K, [&](size_t i) { std::memcpy(wst + woff[i], items[i].witness, woff[i + 1] - woff[i]); });
-    size_t off = 0;
+    // Staging units of at most kStageUnit bytes, in witness order: host
+    // threads copy them into pinned memory while this thread issues each
+    // unit's host-to-device copy as soon as it is staged.
+    struct Unit {
+        size_t item, at, bytes, stage_off;
+    };
+    std::vector<Unit> units;
+    {
+        size_t so = 0;
+        for (size_t i = 0; i < K; ++i) {
+            const size_t b = (size_t)items[i].ctx->cols_ * sizeof(fr_t);
+            for (size_t at = 0; at < b; at += kStageUnit) {
+                units.push_back({i, at, std::min(kStageUnit, b - at), so});
+                so += units.back().bytes;
+            }
+        }
+    }
     for (size_t i = 0; i < K; ++i) {
         const Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
@@ -145,9 +172,29 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         sl.pa.ensure(g.rows_);
         sl.pb.ensure(g.rows_);
         sl.pc.ensure(g.rows_);
-        sl.planes1.ensure(3 * g.wplan_.c + g.hplan_.c);
-        sl.planes2.ensure(g.wplan_.c);
-        NKR_HIP(hipMemcpyAsync(sl.z.get(), wst + woff[i], woff[i + 1] - woff[i], hipMemcpyHostToDevice, stream_));
+    }
+    {
+        const size_t U = units.size();
+        std::unique_ptr<std::atomic<uint8_t>[]> staged(new std::atomic<uint8_t>[U]);
+        for (size_t u = 0; u < U; ++u) staged[u].store(0, std::memory_order_relaxed);
+        std::thread stager([&] {
+            parallel_for(U, [&](size_t u) {
+                const Unit &un = units[u];
+                std::memcpy(wst + un.stage_off, static_cast<const uint8_t *>(items[un.item].witness) + un.at,
+                            un.bytes);
+                staged[u].store(1, std::memory_order_release);
+            });
+        });
+        struct Join {
+            std::thread &t;
+            ~Join() { t.join(); }
+        } join{stager};
+        for (size_t u = 0; u < U; ++u) {
+            while (!staged[u].load(std::memory_order_acquire)) std::this_thread::yield();
+            const Unit &un = units[u];
+            NKR_HIP(hipMemcpyAsync(reinterpret_cast<uint8_t *>(slots_[un.item]->z.get()) + un.at, wst + un.stage_off,
+                                   un.bytes, hipMemcpyHostToDevice, stream_));
+        }
     }
     witness_stage_.uploaded(stream_);
 
@@ -161,7 +208,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
             Groth16Prover &g = *items[i].ctx;
             Slot &sl = *slots_[i];
             wsorts.push_back({&sl.wsched, sl.z.get(), batch_plan(g.wplan_)});
-            g2jobs.push_back({&g.b2_, &sl.wsched, sl.planes2.get()});
+            g2jobs.push_back({&g.b2_, &sl.wsched, planes_2_.get() + p2_off[i]});
         }
         sort_w_.build(wsorts, stream2_);
         NKR_HIP(hipEventRecord(ev_wsched_, stream2_));
@@ -191,7 +238,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         size_t all3, quot;  // byte offsets into tables_
     };
     std::vector<ClassTables> ctabs;
-    off = align16(K * sizeof(SpmvJob));
+    size_t off = align16(K * sizeof(SpmvJob));
     for (const auto &cl : classes) {
         const size_t k = cl.second.size();
         fr_t **p = reinterpret_cast<fr_t **>(tst + off);
@@ -236,30 +283,26 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         Groth16Prover &g = *items[i].ctx;
         Slot &sl = *slots_[i];
         const uint32_t cw = g.wplan_.c;
-        g1jobs.push_back({&g.a_, &sl.wsched, sl.planes1.get() + 0 * cw});
-        g1jobs.push_back({&g.b1_, &sl.wsched, sl.planes1.get() + 1 * cw});
-        g1jobs.push_back({&g.l_, &sl.wsched, sl.planes1.get() + 2 * cw});
-        g1jobs.push_back({&g.h_, &sl.hsched, sl.planes1.get() + 3 * cw});
+        Xyzz<fq_t> *pw = planes_w_.get() + pw_off[i];
+        g1jobs.push_back({&g.a_, &sl.wsched, pw + 0 * cw});
+        g1jobs.push_back({&g.b1_, &sl.wsched, pw + 1 * cw});
+        g1jobs.push_back({&g.l_, &sl.wsched, pw + 2 * cw});
+        g1jobs.push_back({&g.h_, &sl.hsched, planes_h_.get() + ph_off[i]});
     }
     batch_g1_.run(g1jobs, stream_);
 
-    // bit planes of every proof to the host
-    uint8_t *ph = planes_host_.reuse(pbytes);
-    std::vector<size_t> poff(K);
-    off = 0;
-    for (size_t i = 0; i < K; ++i) {
-        const Groth16Prover &g = *items[i].ctx;
-        Slot &sl = *slots_[i];
-        const size_t b1 = (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>);
-        const size_t b2 = g.wplan_.c * sizeof(Xyzz<fq2_t>);
-        poff[i] = off;
-        NKR_HIP(hipMemcpyAsync(ph + off, sl.planes1.get(), b1, hipMemcpyDeviceToHost, stream_));
-        NKR_HIP(hipMemcpyAsync(ph + off + b1, sl.planes2.get(), b2, hipMemcpyDeviceToHost, stream2_));
-        off += b1 + b2;
-    }
-    NKR_HIP(hipEventRecord(ev_g2_, stream2_));
-    NKR_HIP(hipStreamWaitEvent(stream_, ev_g2_, 0));
-
+    // bit planes of every proof to the host: three copies
+    const size_t nw = pw_off[K] * sizeof(Xyzz<fq_t>), nh = ph_off[K] * sizeof(Xyzz<fq_t>);
+    const size_t n2 = p2_off[K] * sizeof(Xyzz<fq2_t>);
+    uint8_t *ph = planes_host_.reuse(nw + nh + n2);
+    NKR_HIP(hipMemcpyAsync(ph, planes_w_.get(), nw, hipMemcpyDeviceToHost, stream_));
+    NKR_HIP(hipMemcpyAsync(ph + nw, planes_h_.get(), nh, hipMemcpyDeviceToHost, stream_));
+    NKR_HIP(hipMemcpyAsync(ph + nw + nh, planes_2_.get(), n2, hipMemcpyDeviceToHost, stream2_));
+    const Xyzz<fq_t> *hw = reinterpret_cast<const Xyzz<fq_t> *>(ph);
+    const Xyzz<fq_t> *hh = reinterpret_cast<const Xyzz<fq_t> *>(ph + nw);
+    const Xyzz<fq2_t> *h2 = reinterpret_cast<const Xyzz<fq2_t> *>(ph + nw + nh);
+    NKR_HIP(hipEventRecord(ev_side_, stream2_));
+    NKR_HIP(hipStreamWaitEvent(stream_, ev_side_, 0));
     // r,s-only blinding terms while the GPU works, then the host tails
     std::vector<Groth16Prover::BlindPre> pre(K);
     parallel_for(K, [&](size_t i) { pre[i] = items[i].ctx->blind_pre(items[i].r, items[i].s); });
@@ -267,11 +310,8 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     parallel_for(K, [&](size_t i) {
         const BatchItem &it = items[i];
         const Groth16Prover &g = *it.ctx;
-        const Xyzz<fq_t> *p1 = reinterpret_cast<const Xyzz<fq_t> *>(ph + poff[i]);
-        const Xyzz<fq2_t> *p2 = reinterpret_cast<const Xyzz<fq2_t> *>(
-            ph + poff[i] + (3 * g.wplan_.c + g.hplan_.c) * sizeof(Xyzz<fq_t>));
         const Groth16Prover::BlindJob job{it.r, it.s, it.proof_out};
-        g.finish(p1, p2, &job, &pre[i], it.raw_out);
+        g.finish(hw + pw_off[i], hh + ph_off[i], h2 + p2_off[i], &job, &pre[i], it.raw_out);
     });
 }
 

@@ -95,14 +95,16 @@ private:
     };
     void run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind = nullptr);
     BlindPre blind_pre(const fr_t &r, const fr_t &s) const;
-    void finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
-                void *raw_out) const;
+    // p1w: the bit planes of ar, bs1 and zk (3 x wplan c), p1h: those of H
+    // (hplan c), p2: those of bs2 (wplan c).
+    void finish(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const Xyzz<fq2_t> *p2, const BlindJob *blind,
+                const BlindPre *pre, void *raw_out) const;
     // The two halves of finish: bs2 (and B) from the G2 planes, the rest from the G1 planes.
     void finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre, void *raw_out) const;
     // helpers: two host threads to split the G1 tail over (null: all on
     // the calling thread, as the batch path's per-proof workers do).
-    void finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out,
-                   HostHelpers *helpers) const;
+    void finish_g1(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const BlindJob *blind, const BlindPre *pre,
+                   void *raw_out, HostHelpers *helpers) const;
     friend class BatchProver;
 
     // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)

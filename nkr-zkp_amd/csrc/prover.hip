@@ -283,11 +283,11 @@ Groth16Prover::BlindPre Groth16Prover::blind_pre(const fr_t &r, const fr_t &s) c
 
 // Host tail: sum_j 2^j U_j for the five MSMs, the raw proof, then the
 // blinded proof from the precomputed r,s-only terms.
-void Groth16Prover::finish(const Xyzz<fq_t> *p1, const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
-                           void *raw_out) const
+void Groth16Prover::finish(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const Xyzz<fq2_t> *p2,
+                           const BlindJob *blind, const BlindPre *pre, void *raw_out) const
 {
     finish_g2(p2, blind, pre, raw_out);
-    finish_g1(p1, blind, pre, raw_out, nullptr);
+    finish_g1(p1w, p1h, blind, pre, raw_out, nullptr);
 }
 
 void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
@@ -300,8 +300,8 @@ void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, cons
     std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 96, &B, 192);
 }
 
-void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const BlindPre *pre, void *raw_out,
-                              HostHelpers *helpers) const
+void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const Xyzz<fq_t> *p1h, const BlindJob *blind,
+                              const BlindPre *pre, void *raw_out, HostHelpers *helpers) const
 {
     const uint32_t cw = wplan_.c;
     uint8_t *out = static_cast<uint8_t *>(raw_out);
@@ -310,7 +310,7 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const BlindJob *blind, const
     g1_xyzz s_ar = g1_xyzz::infinity(), r_bs1 = g1_xyzz::infinity();
     auto part_zq = [&] {
         zk = msm_finish(p1 + 2 * cw, cw);
-        qz = msm_finish(p1 + 3 * cw, hplan_.c);
+        qz = msm_finish(p1h, hplan_.c);
     };
     auto part_b = [&] {
         bs1 = msm_finish(p1 + 1 * cw, cw);
@@ -441,7 +441,7 @@ void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
     planes1_host_.wait();
     host0 = std::chrono::steady_clock::now();
-    finish_g1(h1, blind, blind ? &pre : nullptr, raw_out, helpers_.get());
+    finish_g1(h1, h1 + 3 * cw, blind, blind ? &pre : nullptr, raw_out, helpers_.get());
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - host0).count();
 
     if (t) {

@@ -10,7 +10,8 @@ uint64_t Server::request_bytes(const Groth16Prover &g)
 {
     const uint64_t polys = (uint64_t)g.cols() * sizeof(fr_t) + 3ull * g.rows() * sizeof(fr_t);
     const MsmPlan w = batch_plan(g.witness_plan()), h = batch_plan(g.h_plan());
-    return polys + msm_schedule_bytes(w) + msm_schedule_bytes(h) + 3 * msm_job_bytes<fq_t>(w) +
+    const uint64_t planes = (3ull * w.c + h.c) * sizeof(Xyzz<fq_t>) + (uint64_t)w.c * sizeof(Xyzz<fq2_t>);
+    return polys + planes + msm_schedule_bytes(w) + msm_schedule_bytes(h) + 3 * msm_job_bytes<fq_t>(w) +
            msm_job_bytes<fq_t>(h) + msm_job_bytes<fq2_t>(w);
 }
 
