@@ -1,10 +1,13 @@
 // Radix-2 number-theoretic transforms over Fr for one power-of-two domain.
 //
-// A transform of N = 2^n points runs as 1-3 passes.  Each pass performs
+// A transform of N = 2^n points runs as 1-4 passes.  Each pass performs
 // 2^np-point sub-transforms entirely in LDS (one workgroup holds up to 2048
 // elements = 64 KiB) and multiplies by the inter-pass twiddles omega^(lo*k*2^S)
-// (the four-step factorisation), which come from two small tables
-// (omega^e = hi[e >> m] * lo[e & (2^m - 1)]) that stay cache resident.
+// (the four-step factorisation).  A domain that owns its tables holds them
+// whole, one element per position of each twiddled pass (read like the data,
+// one product per element); an arena domain, whose memory is bounded, makes
+// each from two small tables (omega^e = hi[e >> m] * lo[e & (2^m - 1)]: one
+// more product).
 //
 // dif(): natural order in, bit-reversed order out (decimation in frequency).
 // dit(): bit-reversed order in, natural order out; it is the transpose of
@@ -103,6 +106,10 @@ private:
     uint32_t lo_count_ = 0, hi_count_ = 0, root_count_ = 0;
     int arena_kind_ = -1;  // the kind an arena domain serves; -1 when the tables are owned
     DeviceBuffer<fr_t> tw_lo_[2], tw_hi_[2], roots_[2];
+    // whole inter-pass twiddle tables per direction and pass (none for the
+    // last pass, which has no twiddle): element (j << log_L) + lo holds
+    // omega^((lo * bitrev(j)) << S)
+    std::vector<DeviceBuffer<fr_t>> tw_full_[2];
     DeviceBuffer<fr_t> coset_nat_, coset_rev_, icoset_rev_;
     // The tables in use: the buffers above, or places in an arena (null when absent).
     const fr_t *tw_lo_p_[2] = {}, *tw_hi_p_[2] = {}, *roots_p_[2] = {};

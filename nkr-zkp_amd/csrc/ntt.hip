@@ -30,6 +30,7 @@ struct PassArgs {
     fr_t *const *batch;  // when set, workgroup row blockIdx.y transforms batch[blockIdx.y]
     const fr_t *tw_lo;
     const fr_t *tw_hi;
+    const fr_t *tw_full;  // the pass's whole twiddle table, or null (two-level tables)
     const fr_t *roots;
     const fr_t *pre;
     const fr_t *post;
@@ -50,6 +51,14 @@ __device__ __forceinline__ fr_t tw_pow(const PassArgs &a, uint32_t e)
     const uint32_t lo = e & ((1u << a.m_lo) - 1u);
     const uint32_t hi = e >> a.m_lo;
     return fp_mul(a.tw_hi[hi], a.tw_lo[lo]);
+}
+
+// The inter-pass twiddle of element j of the sub-transform at column lo:
+// omega^e, e = (lo * bitrev(j)) << S, from the whole table when the domain has
+// one.
+__device__ __forceinline__ fr_t tw_at(const PassArgs &a, uint32_t j, uint32_t lo, uint32_t e)
+{
+    return a.tw_full ? a.tw_full[((uint64_t)j << a.log_L) + lo] : tw_pow(a, e);
 }
 
 // Lazy reduction: inside a lazily reduced pass every value is kept below 2r
@@ -251,7 +260,7 @@ __global__ __launch_bounds__(kMaxThreads, RMAX == 2 ? 4 : 1) void k_ntt_pass(Pas
         if (a.pre) v = kLazyNtt<DIT> ? mul_lazy(v, a.pre[pos]) : fp_mul(v, a.pre[pos]);
         if (DIT && a.twiddle) {
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
-            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_pow(a, e)) : fp_mul(v, tw_pow(a, e));
+            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_at(a, j, lo, e)) : fp_mul(v, tw_at(a, j, lo, e));
         }
         lds_put(smem, total, j * cols + c, v);
     }
@@ -302,7 +311,7 @@ __global__ __launch_bounds__(kMaxThreads, RMAX == 2 ? 4 : 1) void k_ntt_pass(Pas
         fr_t v = lds_get(smem, total, j * cols + c);
         if (!DIT && a.twiddle) {
             const uint32_t e = (lo * brev_bits(j, a.np)) << a.S;
-            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_pow(a, e)) : fp_mul(v, tw_pow(a, e));
+            if (e) v = kLazyNtt<DIT> ? mul_lazy(v, tw_at(a, j, lo, e)) : fp_mul(v, tw_at(a, j, lo, e));
         }
         if (a.post) v = kLazyNtt<DIT> ? mul_lazy(v, a.post[pos]) : fp_mul(v, a.post[pos]);
         if (a.has_post_const) v = kLazyNtt<DIT> ? mul_lazy(v, a.post_const) : fp_mul(v, a.post_const);
@@ -350,6 +359,18 @@ __global__ void k_pow_two_level(fr_t *out, const fr_t *__restrict__ hi, const fr
     fr_t v = fp_mul(hi[e >> m], lo[e & ((1u << m) - 1u)]);
     if (has_scale) v = fp_mul(v, scale);
     out[j] = v;
+}
+
+// out[(j << log_L) + lo] = omega^((lo * bitrev(j, np)) << S) for j < 2^np,
+// lo < 2^log_L, from the two-level tables (one product per element).
+__global__ void k_tw_full(fr_t *out, const fr_t *__restrict__ hi, const fr_t *__restrict__ lo_t, uint32_t m,
+                          uint32_t np, uint32_t log_L, uint32_t S)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t(1) << (np + log_L))) return;
+    const uint32_t j = (uint32_t)(i >> log_L), lo = (uint32_t)(i & ((1u << log_L) - 1u));
+    const uint32_t e = (lo * brev_bits(j, np)) << S;
+    out[i] = fp_mul(hi[e >> m], lo_t[e & ((1u << m) - 1u)]);
 }
 
 __global__ void k_bitrev(const fr_t *__restrict__ in, fr_t *__restrict__ out, uint32_t log_n)
@@ -505,6 +526,16 @@ NttDomain::NttDomain(unsigned log_n, hipStream_t stream, const NttShape *shape) 
         tw_lo_p_[d] = tw_lo_[d].get();
         tw_hi_p_[d] = tw_hi_[d].get();
         roots_p_[d] = roots_[d].get();
+        // whole twiddle tables of the passes that have a twiddle (all but the last)
+        tw_full_[d].resize(passes_.size());
+        for (size_t pi = 0; pi + 1 < passes_.size(); ++pi) {
+            const Pass &p = passes_[pi];
+            const size_t count = size_t(1) << (p.np + p.log_L);
+            tw_full_[d][pi].allocate(count);
+            k_tw_full<<<(uint32_t)div_up(count, 256), 256, 0, stream>>>(tw_full_[d][pi].get(), tw_hi_p_[d],
+                                                                      tw_lo_p_[d], m_lo_, p.np, p.log_L, p.S);
+            NKR_LAUNCH_CHECK();
+        }
     }
     coset_nat_.allocate(n);
     coset_rev_.allocate(n);
@@ -567,6 +598,7 @@ void NttDomain::run(fr_t *data, fr_t *const *batch, uint32_t count, bool dit, bo
         a.batch = batch;
         a.tw_lo = tw_lo_p_[d];
         a.tw_hi = tw_hi_p_[d];
+        a.tw_full = (size_t)pi < tw_full_[d].size() ? tw_full_[d][pi].get() : nullptr;
         a.roots = roots_p_[d];
         a.pre = (k == 0) ? pre : nullptr;
         a.post = (k == P - 1) ? post : nullptr;
