@@ -10,9 +10,11 @@
 //   P = U2 + 8p - X, R = S2 + 8p - Y                               < 10p
 //   PP = P^2, RR = R^2, PPP = P PP, Q = X PP  (at most 10p x 10p)  < 2p
 //   X3 = (RR + 2p - PPP) + 4p - 2Q                                 < 8p
-//   Y3 = R (Q + 8p - X3) + 2p - Y PPP                              < 4p
+//   Y3 = R (Q + 8p - X3) + Y (2p - PPP), one Montgomery reduction
+//        for both products (10p x 10p + 8p x 2p = 116 p^2)         < 2p
 //   ZZ3 = ZZ PP, ZZZ3 = ZZZ PPP                                    < 2p
-// so the invariant holds without a single reduction.  A product's inputs stay
+// so the invariant holds without a single reduction.  PP and RR are
+// squarings (fp29_sqr: the symmetric partial products once).  A product's inputs stay
 // below a*b < 2^261 p (fp29_mul).  The exceptional cases (P = 0: the base
 // equals the accumulator, or its negation) take the canonical 32-bit path.
 // Every result is the same field element as xyzz_add_affine's, so the
@@ -90,17 +92,17 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
     // the accumulation runs at three waves per SIMD only while the working
     // set stays near 150 registers (pairing the products, or holding the next
     // base in registers, measured slower on MI355X).
-    const Fp29<P> PP = fp29_mul(Pd, Pd);
+    const Fp29<P> PP = fp29_sqr(Pd);
     const Fp29<P> PPP = fp29_mul(Pd, PP);
-    const Fp29<P> RR = fp29_mul(R, R);
+    const Fp29<P> RR = fp29_sqr(R);
     const Fp29<P> Q = fp29_mul(acc.X, PP);
     Xyzz29<P> r;
     r.ZZ = fp29_mul(acc.ZZ, PP);
     r.X = fp29_sub<P, 4>(fp29_sub<P, 2>(RR, PPP), fp29_add(Q, Q));
-    const Fp29<P> t1 = fp29_mul(R, fp29_sub<P, 8>(Q, r.X));
-    const Fp29<P> t2 = fp29_mul(acc.Y, PPP);
+    const Fp29<P> T = fp29_sub<P, 8>(Q, r.X);
+    const Fp29<P> N = fp29_sub<P, 2>(Fp29<P>::zero(), PPP);
     r.ZZZ = fp29_mul(acc.ZZZ, PPP);
-    r.Y = fp29_sub<P, 2>(t1, t2);
+    r.Y = fp29_mul_add(R, T, acc.Y, N);
     return r;
 }
 
@@ -119,7 +121,9 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
 //   PP = P^2, RR = R^2                                      below 2.3p
 //   PPP = P PP, Q = X PP, ZZ3 = ZZ PP                       below 5p
 //   X3 = (RR + 5p - PPP) + 10p - 2Q, reduced                below 2p
-//   Y3 = R (Q + 2p - X3) + 5p - Y PPP, reduced              below 2p
+//   Y3 = R (Q + 2p - X3) + Y (5p - PPP), one reduction per
+//        Karatsuba term (w2: 8p x 14p + 4p x 10p = 152 p^2),
+//        reduced                                            below 2p
 //   ZZZ3 = ZZZ PPP                                          below 5p
 template <class P>
 struct Fq2_29 {
@@ -146,6 +150,21 @@ NKR_HD Fq2_29<P> fq2_29_mul(const Fq2_29<P> &a, const Fq2_29<P> &b)
     const Fp29<P> v1 = fp29_mul(a.c1, b.c1);
     const Fp29<P> v2 = fp29_mul(fp29_add(a.c0, a.c1), fp29_add(b.c0, b.c1));
     return {fp29_sub<P, 2>(v0, v1), fp29_sub<P, 3>(v2, fp29_add(v0, v1))};
+}
+
+// a*b + c*d with one Montgomery reduction per Karatsuba term (fp29_mul_add):
+// w0 = a0 b0 + c0 d0, w1 = a1 b1 + c1 d1, w2 = (a0 + a1)(b0 + b1) +
+// (c0 + c1)(d0 + d1); result (w0 - w1, w2 - w0 - w1).  Needs the w2 inputs
+// below 169 p^2 together (each w below 2p; components: c0 below w0 + 2p,
+// c1 below w2 + 3p).
+template <class P>
+NKR_HD Fq2_29<P> fq2_29_mul_add(const Fq2_29<P> &a, const Fq2_29<P> &b, const Fq2_29<P> &c, const Fq2_29<P> &d)
+{
+    const Fp29<P> w0 = fp29_mul_add(a.c0, b.c0, c.c0, d.c0);
+    const Fp29<P> w1 = fp29_mul_add(a.c1, b.c1, c.c1, d.c1);
+    const Fp29<P> w2 =
+        fp29_mul_add(fp29_add(a.c0, a.c1), fp29_add(b.c0, b.c1), fp29_add(c.c0, c.c1), fp29_add(d.c0, d.c1));
+    return {fp29_sub<P, 2>(w0, w1), fp29_sub<P, 3>(w2, fp29_add(w0, w1))};
 }
 
 template <class P>
@@ -249,10 +268,11 @@ NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &
     const Fq2_29<P> Q2 = {fp29_add(Q.c0, Q.c0), fp29_add(Q.c1, Q.c1)};
     r.X = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(fq2_29_sub_if_ge<P, 8>(
         fq2_29_sub_if_ge<P, 16>(fq2_29_sub<P, 10>(fq2_29_sub<P, 5>(RR, PPP), Q2)))));
-    const Fq2_29<P> t1 = fq2_29_mul(R, fq2_29_sub<P, 2>(Q, r.X));
-    const Fq2_29<P> t2 = fq2_29_mul(acc.Y, PPP);
+    const Fq2_29<P> T = fq2_29_sub<P, 2>(Q, r.X);
+    const Fq2_29<P> N = fq2_29_sub<P, 5>({Fp29<P>::zero(), Fp29<P>::zero()}, PPP);
     r.ZZZ = fq2_29_mul(acc.ZZZ, PPP);
-    r.Y = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(fq2_29_sub_if_ge<P, 8>(fq2_29_sub<P, 5>(t1, t2))));
+    const Fq2_29<P> y = fq2_29_mul_add(R, T, acc.Y, N);  // c0 below 3.3p, c1 below 5p
+    r.Y = {fp29_sub_if_ge<P, 2>(y.c0), fp29_sub_if_ge<P, 2>(fp29_sub_if_ge<P, 4>(y.c1))};
     return r;
 }
 

@@ -359,6 +359,82 @@ NKR_HD Fp29<P> fp29_mul(const Fp29<P> &a, const Fp29<P> &b)
     return r;
 }
 
+// a*a*2^-261 mod p, the same value and bounds as fp29_mul(a, a): each
+// symmetric pair of partial products is formed once from a doubled limb
+// (2 a_i < 2^30, so a pair term stays below 2^59 and a column below 2^63),
+// 45 multiply-adds for the product half instead of 81.
+template <class P>
+NKR_HD Fp29<P> fp29_sqr(const Fp29<P> &a)
+{
+    constexpr uint32_t INV = P29<P>::inv();
+    uint32_t d[9], m[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) d[i] = a.l[i] << 1;
+    Fp29<P> r;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; 2 * i < k; ++i) acc += (uint64_t)d[i] * a.l[k - i];
+        if (k % 2 == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        m[k] = ((uint32_t)acc * INV) & kMask29;
+        acc += (uint64_t)m[k] * P29<P>::bits(0);
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; 2 * i < k; ++i) acc += (uint64_t)d[i] * a.l[k - i];
+        if (k % 2 == 0) acc += (uint64_t)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        r.l[k - 9] = (uint32_t)acc & kMask29;
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
+// (a*b + c*d)*2^-261 mod p with one Montgomery reduction: both products'
+// partial sums go into the same columns (27 terms below 2^58 each, below
+// 2^63).  Inputs: limbs below 2^29, a*b + c*d < 2^261 * p.  Output: limbs
+// below 2^29, value below p + (a*b + c*d)/2^261 < 2p.
+template <class P>
+NKR_HD Fp29<P> fp29_mul_add(const Fp29<P> &a, const Fp29<P> &b, const Fp29<P> &c, const Fp29<P> &d)
+{
+    constexpr uint32_t INV = P29<P>::inv();
+    uint32_t m[9];
+    Fp29<P> r;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = 0; i <= k; ++i) acc += (uint64_t)c.l[i] * d.l[k - i];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        m[k] = ((uint32_t)acc * INV) & kMask29;
+        acc += (uint64_t)m[k] * P29<P>::bits(0);
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)a.l[i] * b.l[k - i];
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)c.l[i] * d.l[k - i];
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        r.l[k - 9] = (uint32_t)acc & kMask29;
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
 // Two independent products r0 = a0*b0 and r1 = a1*b1 (fp29_mul's bounds for
 // each) with their multiply-adds interleaved: each product's column sum is
 // one dependent chain through its accumulator, so a wave that alternates two

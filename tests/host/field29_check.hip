@@ -5,6 +5,8 @@
 //     29-bit Montgomery one converts to the canonical one;
 //   * products: every input bound the formulas use (below 2p, 10p, and 32p
 //     against 2p), built as x + k*p, gives fp_mul's canonical result;
+//     squarings give fp29_mul(a, a)'s limbs, and a sum of two products with
+//     one reduction the canonical sum;
 //   * lazy sums and differences: a + b and a + K*p - b are congruent to the
 //     canonical ones;
 //   * mixed addition: chains of xyzz29_add_affine over random bases, repeated
@@ -74,6 +76,21 @@ static void check_field(int n)
                "mul 32p x 2p", i);
         expect(fp29_to_mont(fp29_mul(widen<P, 8>(x), widen<P, 8>(y))) == want, "mul 10p x 10p", i);
         expect(fp29_to_mont(fp29_mul(widen<P, 0>(x), widen<P, 0>(y))) == want, "mul 2p x 2p", i);
+        expect(fp29_to_mont(fp29_sqr(widen<P, 8>(x))) == fp_mul(x, x), "sqr 10p", i);
+        {
+            const Fp29<P> wx = widen<P, 8>(x), s0 = fp29_sqr(wx), s1 = fp29_mul(wx, wx);
+            bool same = true;
+            for (int k = 0; k < 9; ++k) same = same && s0.l[k] == s1.l[k];
+            expect(same, "sqr limbs equal mul(a, a)", i);
+        }
+        {
+            // the G1 Y3 shape: 10p x 10p + 8p x 2p in one reduction
+            const Fp<P> u = rand_fp<P>(), v = rand_fp<P>();
+            const Fp29<P> s = fp29_mul_add(widen<P, 8>(x), widen<P, 8>(y), widen<P, 6>(u), widen<P, 0>(v));
+            bool limbs = true;
+            for (int k = 0; k < 9; ++k) limbs = limbs && (s.l[k] >> 29) == 0;
+            expect(limbs && fp29_to_mont(s) == fp_add(fp_mul(x, y), fp_mul(u, v)), "mul_add", i);
+        }
         const Fp29<P> m = fp29_mul(widen<P, 8>(x), widen<P, 8>(y));
         bool limbs = true;
         for (int k = 0; k < 9; ++k) limbs = limbs && (m.l[k] >> 29) == 0;
@@ -154,6 +171,12 @@ static void check_fq2(int n)
         expect(fq2_29_to(fq2_29_mul(widen2<6>(a), widen2<3>(b))) == want, "fq2 mul 8p x 5p", i);
         expect(fq2_29_to(fq2_29_mul(widen2<3>(a), widen2<6>(b))) == want, "fq2 mul 5p x 8p", i);
         expect(fq2_29_to(fq2_29_sqr(widen2<3>(a))) == fq2_sqr(a), "fq2 sqr 5p", i);
+        {
+            // the G2 Y3 shape: components 4p x 7p + 2p x 5p
+            const fq2_t c = rand_fq2(), d = rand_fq2();
+            const Fq2_29<FqParams> y = fq2_29_mul_add(widen2<2>(a), widen2<5>(b), widen2<0>(c), widen2<3>(d));
+            expect(fq2_29_to(y) == fq2_add(fq2_mul(a, b), fq2_mul(c, d)), "fq2 mul_add", i);
+        }
         const Fq2_29<FqParams> m = fq2_29_mul(widen2<6>(a), widen2<3>(b));
         bool limbs = true;
         for (int k = 0; k < 9; ++k) limbs = limbs && (m.c0.l[k] >> 29) == 0 && (m.c1.l[k] >> 29) == 0;
