@@ -79,8 +79,8 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
     }
     const Fp29<P> U2 = fp29_mul(x2, acc.ZZ);
     const Fp29<P> S2 = fp29_mul(y2, acc.ZZZ);
-    const Fp29<P> Pd = fp29_sub<P, 8>(U2, acc.X);
-    const Fp29<P> R = fp29_sub<P, 8>(S2, acc.Y);
+    Fp29<P> Pd = fp29_sub<P, 8>(U2, acc.X);
+    Fp29<P> R = fp29_sub<P, 8>(S2, acc.Y);
     if (fp29_may_be_zero<P, 10>(Pd) && fp29_to_mont(Pd).is_zero()) {
         // the base is +-(the accumulator): the canonical formulas decide
         // (doubling or infinity)
@@ -92,6 +92,8 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
     // the accumulation runs at three waves per SIMD only while the working
     // set stays near 150 registers (pairing the products, or holding the next
     // base in registers, measured slower on MI355X).
+    fp29_opaque(Pd);
+    fp29_opaque(R);
     const Fp29<P> PP = fp29_sqr(Pd);
     const Fp29<P> PPP = fp29_mul(Pd, PP);
     const Fp29<P> RR = fp29_sqr(R);

@@ -110,6 +110,23 @@ NKR_HD Fp<P> fp29_to_mont(const Fp29<P> &v)
     return fp_reduce_once(r);
 }
 
+// Hides x's provenance from the optimiser without changing it (an empty
+// asm statement per limb, no instruction).  Where a product's operand comes
+// from code the compiler also sees feeding 64-bit arithmetic (the exact zero
+// test of the addition's exceptional case), it may carry the limbs as 64-bit
+// values and square or multiply them as such: on gfx950 that cost the G1
+// bucket addition 135 extra multiply-adds and 260 register moves.
+template <class P>
+NKR_HD void fp29_opaque(Fp29<P> &x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+    for (int i = 0; i < 9; ++i) asm volatile("" : "+v"(x.l[i]));
+#else
+    (void)x;
+#endif
+}
+
 // Could x (below k_max * p, k_max <= 16) be 0 mod p?  A necessary condition
 // on the low limb only: x = j*p for some j < k_max.  Exact answers come from
 // fp29_to_mont(x).is_zero().

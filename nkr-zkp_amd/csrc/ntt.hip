@@ -61,6 +61,19 @@ __device__ __forceinline__ fr_t tw_at(const PassArgs &a, uint32_t j, uint32_t lo
     return a.tw_full ? a.tw_full[((uint64_t)j << a.log_L) + lo] : tw_pow(a, e);
 }
 
+// 2r as eight 32-bit words (r < 2^254, so 2r < 2^255).
+constexpr Fp<FrParams> twice_modulus()
+{
+    Fp<FrParams> t{};
+    uint64_t c = 0;
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t v = 2ull * FrParams::P[i] + c;
+        t.l[i] = (uint32_t)v;
+        c = v >> 32;
+    }
+    return t;
+}
+
 // Lazy reduction: inside a lazily reduced pass every value is kept below 2r
 // (r < 2^254, so 4r < 2^256 and sums of two such values fit the eight words);
 // products skip their final subtraction (below 2r again when the other factor
@@ -69,16 +82,11 @@ __device__ __forceinline__ fr_t tw_at(const PassArgs &a, uint32_t j, uint32_t lo
 // x - 2r if x >= 2r, for x < 4r.
 __device__ __forceinline__ fr_t sub_twice_r_if(const fr_t &x)
 {
+    constexpr Fp<FrParams> two_r = twice_modulus();
     fr_t d;
-    uint64_t borrow = 0, c2 = 0;
+    uint32_t borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t two = 2ull * FrParams::P[i] + c2;
-        c2 = two >> 32;
-        const uint64_t t = (uint64_t)x.l[i] - (uint32_t)two - borrow;
-        d.l[i] = (uint32_t)t;
-        borrow = (t >> 63) & 1;
-    }
+    for (int i = 0; i < 8; ++i) d.l[i] = subc32(x.l[i], two_r.l[i], borrow, borrow);
     fr_t r;
 #pragma unroll
     for (int i = 0; i < 8; ++i) r.l[i] = borrow ? x.l[i] : d.l[i];
@@ -89,35 +97,22 @@ __device__ __forceinline__ fr_t sub_twice_r_if(const fr_t &x)
 __device__ __forceinline__ fr_t add_lazy(const fr_t &x, const fr_t &y)
 {
     fr_t s;
-    uint64_t carry = 0;
+    uint32_t carry = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t t = (uint64_t)x.l[i] + y.l[i] + carry;
-        s.l[i] = (uint32_t)t;
-        carry = t >> 32;
-    }
+    for (int i = 0; i < 8; ++i) s.l[i] = addc32(x.l[i], y.l[i], carry, carry);
     return sub_twice_r_if(s);
 }
 
 // x - y + 2r for x, y < 2r: in (0, 4r), not reduced.
 __device__ __forceinline__ fr_t sub_lazy(const fr_t &x, const fr_t &y)
 {
+    constexpr Fp<FrParams> two_r = twice_modulus();
     fr_t s;
-    uint64_t carry = 0, c2 = 0, borrow = 0;
+    uint32_t carry = 0, borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t two = 2ull * FrParams::P[i] + c2;
-        c2 = two >> 32;
-        const uint64_t t = (uint64_t)x.l[i] + (uint32_t)two + carry;
-        s.l[i] = (uint32_t)t;
-        carry = t >> 32;
-    }
+    for (int i = 0; i < 8; ++i) s.l[i] = addc32(x.l[i], two_r.l[i], carry, carry);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const uint64_t t = (uint64_t)s.l[i] - y.l[i] - borrow;
-        s.l[i] = (uint32_t)t;
-        borrow = (t >> 63) & 1;
-    }
+    for (int i = 0; i < 8; ++i) s.l[i] = subc32(s.l[i], y.l[i], borrow, borrow);
     return s;
 }
 
