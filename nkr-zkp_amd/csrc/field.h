@@ -100,7 +100,7 @@ struct Fp {
 // plus the moves that build their register pairs, about three times the
 // instructions (fp_add: 100 VALU instructions against 35).  Host code keeps
 // the portable form.
-NKR_HD inline uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout)
+NKR_HD uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     unsigned int c;
@@ -114,7 +114,7 @@ NKR_HD inline uint32_t addc32(uint32_t a, uint32_t b, uint32_t cin, uint32_t &co
 #endif
 }
 
-NKR_HD inline uint32_t subc32(uint32_t a, uint32_t b, uint32_t bin, uint32_t &bout)
+NKR_HD uint32_t subc32(uint32_t a, uint32_t b, uint32_t bin, uint32_t &bout)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     unsigned int c;
