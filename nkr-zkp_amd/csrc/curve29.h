@@ -108,6 +108,50 @@ NKR_HD Xyzz29<P> xyzz29_add_affine(const Xyzz29<P> &acc, const Fp29<P> &x2, cons
     return r;
 }
 
+// a + b, complete (add-2008-s), both in the accumulator invariant (X, Y
+// below 8p; ZZ, ZZZ below 2p; infinity ZZ = 0 exactly), for the bucket
+// reduction (fold, completion, bit-plane tree), whose partial sums stay in
+// this form from the accumulation to the bit planes:
+//   U1 = X1 ZZ2, U2 = X2 ZZ1, S1 = Y1 ZZZ2, S2 = Y2 ZZZ1  (8p x 2p)  < 2p
+//   P = U2 + 2p - U1, R = S2 + 2p - S1                           < 4p
+//   PP = P^2, RR = R^2, PPP = P PP, Q = U1 PP                     < 2p
+//   X3 = (RR + 2p - PPP) + 4p - 2Q                                < 8p
+//   Y3 = R (Q + 8p - X3) + S1 (2p - PPP), one reduction
+//        (4p x 10p + 2p x 2p = 44 p^2)                            < 2p
+//   ZZ3 = (ZZ1 ZZ2) PP, ZZZ3 = (ZZZ1 ZZZ2) PPP                    < 2p
+// The exceptional case P = 0 (b = +-a) takes the canonical doubling or
+// gives infinity.
+template <class P>
+NKR_HD Xyzz29<P> xyzz29_add(const Xyzz29<P> &a, const Xyzz29<P> &b)
+{
+    if (a.is_inf()) return b;
+    if (b.is_inf()) return a;
+    const Fp29<P> U1 = fp29_mul(a.X, b.ZZ);
+    const Fp29<P> U2 = fp29_mul(b.X, a.ZZ);
+    const Fp29<P> S1 = fp29_mul(a.Y, b.ZZZ);
+    const Fp29<P> S2 = fp29_mul(b.Y, a.ZZZ);
+    Fp29<P> Pd = fp29_sub<P, 2>(U2, U1);
+    Fp29<P> R = fp29_sub<P, 2>(S2, S1);
+    if (fp29_may_be_zero<P, 4>(Pd) && fp29_to_mont(Pd).is_zero()) {
+        if (fp29_to_mont(R).is_zero()) return xyzz29_from<P>(xyzz_dbl(xyzz29_to(a)));
+        return Xyzz29<P>::infinity();
+    }
+    fp29_opaque(Pd);
+    fp29_opaque(R);
+    const Fp29<P> PP = fp29_sqr(Pd);
+    const Fp29<P> PPP = fp29_mul(Pd, PP);
+    const Fp29<P> Q = fp29_mul(U1, PP);
+    const Fp29<P> RR = fp29_sqr(R);
+    Xyzz29<P> r;
+    r.ZZ = fp29_mul(fp29_mul(a.ZZ, b.ZZ), PP);
+    r.ZZZ = fp29_mul(fp29_mul(a.ZZZ, b.ZZZ), PPP);
+    r.X = fp29_sub<P, 4>(fp29_sub<P, 2>(RR, PPP), fp29_add(Q, Q));
+    const Fp29<P> T = fp29_sub<P, 8>(Q, r.X);
+    const Fp29<P> N = fp29_sub<P, 2>(Fp29<P>::zero(), PPP);
+    r.Y = fp29_mul_add(R, T, S1, N);
+    return r;
+}
+
 // ---------------------------------------------------------------- G2 (Fq2)
 // Fq2 = Fq[u]/(u^2 + 1) over the 29-bit form.  fp29_mul needs a*b < 2^261 p
 // = 169.28 p^2; a Karatsuba product multiplies the sums of the components,

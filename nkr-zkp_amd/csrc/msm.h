@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "curve.h"
+#include "curve29.h"
 #include "msm_plan.h"
 #include "runtime.h"
 
@@ -178,6 +179,21 @@ private:
     PinnedStaging<uint32_t> totals_host_;
 };
 
+// The form of the partial sums between the accumulation and the bit planes:
+// G1 keeps the reduced-radix form (curve29.h) through fold, completion and
+// tree, so no addition converts; G2 partial sums are canonical XYZZ.  The
+// bit planes handed to the host are canonical either way.
+template <class F>
+struct MsmPartial {
+    using type = Xyzz<F>;
+};
+template <>
+struct MsmPartial<fq_t> {
+    using type = Xyzz29<FqParams>;
+};
+template <class F>
+using partial_t = typename MsmPartial<F>::type;
+
 // Stage indices used with StageTimer by the MSM code.
 enum MsmStage { kStageSort = 3, kStageAccumulate = 4, kStageFold = 5, kStageAccumulateG2 = 7 };
 
@@ -232,7 +248,7 @@ private:
     DeviceBuffer<uint8_t> jobs_dev_;
     DeviceBuffer<uint32_t> thread_base_dev_;
     PinnedStaging<uint8_t> jobs_stage_;
-    DeviceBuffer<Xyzz<F>> pool_, sums_, tree_[2];
+    DeviceBuffer<partial_t<F>> pool_, sums_, tree_[2];
     DeviceBuffer<uint32_t> reg_start_, reg_count_, fold_start_;
 };
 

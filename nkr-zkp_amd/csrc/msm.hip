@@ -40,17 +40,29 @@ struct JobDev {
     const uint32_t *piece_start;   // B + 1
     const uint32_t *piece_max;     // largest level-0 piece count of any bucket
     const uint32_t *piece_len;     // entries per level-0 piece (chosen on the device)
-    Xyzz<F> *pool;                 // partial sums (level 0 region, then fold levels)
+    partial_t<F> *pool;            // partial sums (level 0 region, then fold levels)
     uint32_t *reg_start;           // B: where bucket k's partials live in `pool`
     uint32_t *reg_count;           // B
     uint32_t *fold_start;          // B + 1
-    Xyzz<F> *sums;                 // B: S_{k+1}
-    Xyzz<F> *tree[2];              // B each
-    Xyzz<F> *planes;               // c
+    partial_t<F> *sums;            // B: S_{k+1}
+    partial_t<F> *tree[2];         // B each
+    Xyzz<F> *planes;               // c (canonical)
     uint32_t B, c;
     uint32_t list_off;                    // this job's completion lists start at 4 * list_off
     uint32_t level_base[kMaxFoldLevels];  // where each fold level's partials start in `pool`
 };
+
+// Sum of two partial sums, and a partial sum as a canonical bit plane.
+__device__ __forceinline__ Xyzz29<FqParams> partial_add(const Xyzz29<FqParams> &a, const Xyzz29<FqParams> &b)
+{
+    return xyzz29_add(a, b);
+}
+__device__ __forceinline__ Xyzz<fq2_t> partial_add(const Xyzz<fq2_t> &a, const Xyzz<fq2_t> &b)
+{
+    return xyzz_add(a, b);
+}
+__device__ __forceinline__ Xyzz<fq_t> partial_plane(const Xyzz29<FqParams> &a) { return xyzz29_to(a); }
+__device__ __forceinline__ Xyzz<fq2_t> partial_plane(const Xyzz<fq2_t> &a) { return a; }
 
 // ------------------------------------------------------------ digit recoding
 // Signed c-bit digits of an integer scalar (< 2^254): d_w in [-2^(c-1), 2^(c-1)],
@@ -430,7 +442,7 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
         const uint32_t end = min(beg + piece, jb.bucket_start[k + 1]);
         const Affine<F> *__restrict__ bases = jb.bases;
         const uint32_t *__restrict__ sorted = jb.sorted;
-        Xyzz<F> acc = Xyzz<F>::infinity();
+        partial_t<F> acc = partial_t<F>::infinity();
         if constexpr (std::is_same<F, fq2_t>::value) {
             // G2: the accumulator in the reduced-radix form as for G1
             // (curve29.h, Fq2 bounds); no prefetch.
@@ -448,8 +460,8 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
         } else if (beg < end) {
             // G1: the accumulator stays in the reduced-radix form
             // (curve29.h: one v_mad_u64_u32 per product term, no carries)
-            // for the whole piece; bases are converted as they arrive and
-            // the partial sum once at the end.  The next sorted index is
+            // for the whole piece and is stored in it; bases are converted
+            // as they arrive.  The next sorted index is
             // fetched one entry ahead; the base itself is not (its 16
             // registers cost more occupancy than its latency costs three
             // resident waves, measured on MI355X).
@@ -465,7 +477,7 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
                 }
                 v = vn;
             }
-            acc = xyzz29_to(acc29);
+            acc = acc29;
         }
         jb.pool[p] = acc;
     }
@@ -513,8 +525,8 @@ __global__ __launch_bounds__(256) void k_fold(const JobDev<F> *__restrict__ jobs
     const uint32_t q = p - jb.fold_start[k];
     const uint32_t beg = jb.reg_start[k] + q * kMsmFoldIn;
     const uint32_t end = min(beg + kMsmFoldIn, jb.reg_start[k] + jb.reg_count[k]);
-    Xyzz<F> acc = jb.pool[beg];
-    for (uint32_t e = beg + 1; e < end; ++e) acc = xyzz_add(acc, jb.pool[e]);
+    partial_t<F> acc = jb.pool[beg];
+    for (uint32_t e = beg + 1; e < end; ++e) acc = partial_add(acc, jb.pool[e]);
     jb.pool[jb.level_base[L] + p] = acc;
 }
 
@@ -531,14 +543,14 @@ __global__ __launch_bounds__(256) void k_fold_retarget(const JobDev<F> *__restri
     }
 }
 
-template <class F>
-__device__ __forceinline__ Xyzz<F> shfl_xor_point(const Xyzz<F> &p, int mask)
+template <class T>
+__device__ __forceinline__ T shfl_xor_point(const T &p, int mask)
 {
-    Xyzz<F> r;
+    T r;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(&p);
     uint32_t *dst = reinterpret_cast<uint32_t *>(&r);
 #pragma unroll
-    for (int i = 0; i < (int)(sizeof(Xyzz<F>) / 4); ++i) dst[i] = (uint32_t)__shfl_xor((int)src[i], mask, 16);
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = (uint32_t)__shfl_xor((int)src[i], mask, 16);
     return r;
 }
 
@@ -566,7 +578,7 @@ __global__ __launch_bounds__(256) void k_classify(const JobDev<F> *__restrict__ 
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = k < jb.B;
     const uint32_t m = in ? jb.reg_count[k] : 0u;
-    if (in && m == 0) jb.sums[k] = Xyzz<F>::infinity();
+    if (in && m == 0) jb.sums[k] = partial_t<F>::infinity();
     const uint32_t c = complete_class(m, max_class);
     const uint32_t lane = threadIdx.x & 63u;
     uint32_t *list = class_list + (size_t)jb.list_off * kCompleteClasses;
@@ -631,13 +643,14 @@ __global__ __launch_bounds__(256) void k_complete(const JobDev<F> *__restrict__ 
         const bool live = idx < class_count[s];
         const uint32_t k =
             live ? class_list[(size_t)jb.list_off * kCompleteClasses + (size_t)c * jb.B + idx] : 0u;
-        Xyzz<F> acc = Xyzz<F>::infinity();
+        partial_t<F> acc = partial_t<F>::infinity();
         if (live) {
             const uint32_t start = jb.reg_start[k];
             const uint32_t cnt = jb.reg_count[k];
-            for (uint32_t i = lane; i < cnt; i += (1u << c)) acc = xyzz_add(acc, jb.pool[start + i]);
+            for (uint32_t i = lane; i < cnt; i += (1u << c)) acc = partial_add(acc, jb.pool[start + i]);
         }
-        for (uint32_t off = (1u << c) >> 1; off >= 1; off >>= 1) acc = xyzz_add(acc, shfl_xor_point(acc, (int)off));
+        for (uint32_t off = (1u << c) >> 1; off >= 1; off >>= 1)
+            acc = partial_add(acc, shfl_xor_point(acc, (int)off));
         if (live && lane == 0) jb.sums[k] = acc;
     }
 }
@@ -665,14 +678,15 @@ __global__ __launch_bounds__(256) void k_tree_lds(const JobDev<F> *__restrict__ 
     const uint32_t lv = min(h, jb.c - 1 - t0);
     if (blockIdx.x >= (jb.B >> (t0 + lv))) return;
     const uint32_t cap = (1u << h) * (t0 + 1);
-    Xyzz<F> *buf[2] = {reinterpret_cast<Xyzz<F> *>(tree_lds), reinterpret_cast<Xyzz<F> *>(tree_lds) + cap};
+    partial_t<F> *buf[2] = {reinterpret_cast<partial_t<F> *>(tree_lds),
+                            reinterpret_cast<partial_t<F> *>(tree_lds) + cap};
 
     const uint32_t nin = (1u << lv) * (t0 + 1);
     const uint32_t blk0 = blockIdx.x << lv;
     for (uint32_t idx = threadIdx.x; idx < nin; idx += blockDim.x) {
         const uint32_t blk = blk0 + idx / (t0 + 1), ent = idx % (t0 + 1);
         if (t0 == 0)
-            buf[0][idx] = blk == 0 ? Xyzz<F>::infinity() : jb.sums[blk - 1];
+            buf[0][idx] = blk == 0 ? partial_t<F>::infinity() : jb.sums[blk - 1];
         else
             buf[0][idx] = jb.tree[src][(size_t)blk * (t0 + 1) + ent];
     }
@@ -680,23 +694,23 @@ __global__ __launch_bounds__(256) void k_tree_lds(const JobDev<F> *__restrict__ 
     for (uint32_t s = 0; s < lv; ++s) {
         const uint32_t t = t0 + s, E = t + 2;
         const bool last = (t + 1 == jb.c - 1);
-        const Xyzz<F> *in = buf[s & 1];
-        Xyzz<F> *out = buf[(s + 1) & 1];
+        const partial_t<F> *in = buf[s & 1];
+        partial_t<F> *out = buf[(s + 1) & 1];
         const uint32_t nout = (1u << (lv - s - 1)) * E;
         for (uint32_t idx = threadIdx.x; idx < nout; idx += blockDim.x) {
             const uint32_t b = idx / E, e = idx % E;
-            const Xyzz<F> *l = in + (size_t)(2 * b) * (t + 1);
-            const Xyzz<F> *r = l + (t + 1);
+            const partial_t<F> *l = in + (size_t)(2 * b) * (t + 1);
+            const partial_t<F> *r = l + (t + 1);
             // one addition site for every lane: the P_e and T merges do not
             // diverge into two serial additions
             const uint32_t q = e < t ? e : t;
-            Xyzz<F> o = xyzz_add(l[q], r[q]);
+            partial_t<F> o = partial_add(l[q], r[q]);
             if (e == t)
                 o = r[t];
             else if (e > t && last)
                 o = jb.sums[jb.B - 1];
             if (last)
-                jb.planes[e] = o;
+                jb.planes[e] = partial_plane(o);
             else
                 out[idx] = o;
         }
@@ -784,7 +798,7 @@ uint64_t msm_job_bytes(const MsmPlan &p)
     // partial sums; per bucket: sums and two tree levels, region start,
     // count and fold start, the completion class lists; the bit planes; and
     // the job's fixed records (job table, class counters, piece offsets)
-    return total * sizeof(Xyzz<F>) + (uint64_t)p.B * (3 * sizeof(Xyzz<F>) + 12 + 4 * kCompleteClasses) +
+    return total * sizeof(partial_t<F>) + (uint64_t)p.B * (3 * sizeof(partial_t<F>) + 12 + 4 * kCompleteClasses) +
            p.c * sizeof(Xyzz<F>) + 1024;
 }
 
@@ -1085,8 +1099,9 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     // SIMD's registers
     for (uint32_t t = 0, src = 0; t + 1 <= cmax - 1; src ^= 1u) {
         uint32_t h = 1;
-        while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(Xyzz<F>) <= tree_lds_bytes<F>()) ++h;
-        const size_t lds = 2 * (size_t(1) << h) * (t + 1) * sizeof(Xyzz<F>);
+        while (t + h < cmax - 1 && 2 * (size_t(2) << h) * (t + 1) * sizeof(partial_t<F>) <= tree_lds_bytes<F>())
+            ++h;
+        const size_t lds = 2 * (size_t(1) << h) * (t + 1) * sizeof(partial_t<F>);
         const uint32_t widest = (1u << (h - 1)) * (t + 2);
         const uint32_t lanes = std::min<uint32_t>(256, (uint32_t)div_up(widest, 64) * 64);
         k_tree_lds<F><<<dim3(Bmax >> (t + h), J), lanes, lds, stream>>>(jd, t, h, src);

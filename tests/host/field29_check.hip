@@ -12,6 +12,9 @@
 //   * mixed addition: chains of xyzz29_add_affine over random bases, repeated
 //     bases (doubling), negated ones (infinity) and infinity restarts give,
 //     after every step, the canonical coordinates of xyzz_add_affine;
+//   * full addition: chains of xyzz29_add over partial sums (ZZ != 1),
+//     doublings, cancellations and infinity operands give xyzz_add's
+//     coordinates after every step;
 //   * the same over Fq2 (G2): Karatsuba products and squares at their input
 //     bounds, and chains of xyzz2_29_add_affine against xyzz_add_affine.
 //     The formulas are field identities, so random coordinates (not on the
@@ -154,6 +157,62 @@ static void check_madd(int chains, int len)
     }
 }
 
+// Full additions: sums of random partial sums (each a chain of mixed
+// additions, so ZZ != 1), including a + a (doubling), a + (-a) (infinity)
+// and infinity operands, against xyzz_add after every step.
+static void check_add(int chains, int len)
+{
+    Affine<fq_t> g;
+    g.x = fq_t::zero();
+    g.x.l[0] = 1;
+    g.x = fp_to_mont(g.x);
+    g.y = fq_t::zero();
+    g.y.l[0] = 2;
+    g.y = fp_to_mont(g.y);
+    std::vector<Affine<fq_t>> pool;
+    for (int i = 0; i < 16; ++i) pool.push_back(random_base(g));
+    auto partial = [&](int n, Xyzz<fq_t> &c32, Xyzz29<FqParams> &c29) {
+        c32 = Xyzz<fq_t>::infinity();
+        c29 = Xyzz29<FqParams>::infinity();
+        for (int e = 0; e < n; ++e) {
+            const Affine<fq_t> b = pool[rnd() % pool.size()];
+            c32 = xyzz_add_affine(c32, b.x, b.y);
+            c29 = xyzz29_add_affine(c29, fp29_from_mont(b.x), fp29_from_mont(b.y));
+        }
+    };
+    for (int c = 0; c < chains; ++c) {
+        Xyzz<fq_t> a32;
+        Xyzz29<FqParams> a29;
+        partial(1 + rnd() % 3, a32, a29);
+        for (int e = 0; e < len; ++e) {
+            const uint32_t r = rnd();
+            Xyzz<fq_t> b32;
+            Xyzz29<FqParams> b29;
+            partial((r >> 4) % 4, b32, b29);  // 0: infinity
+            if ((r >> 8) % 6 == 0) {          // a + a
+                b32 = a32;
+                b29 = a29;
+                ++g_doublings;
+            } else if ((r >> 12) % 7 == 0 && !a32.is_inf()) {  // a + (-a)
+                b32 = a32;
+                b32.Y = fp_neg(b32.Y);
+                b29 = a29;
+                b29.Y = fp29_sub<FqParams, 8>(Fp29<FqParams>::zero(), a29.Y);
+                ++g_cancels;
+            }
+            a32 = xyzz_add(a32, b32);
+            a29 = xyzz29_add(a29, b29);
+            const Xyzz<fq_t> got = xyzz29_to(a29);
+            const Affine<fq_t> w = xyzz_to_affine(a32), h = xyzz_to_affine(got);
+            expect(got.is_inf() == a32.is_inf() && (a32.is_inf() || (w.x == h.x && w.y == h.y)), "add chain",
+                   c * len + e);
+            expect(got.X == a32.X && got.Y == a32.Y && got.ZZ == a32.ZZ && got.ZZZ == a32.ZZZ, "add chain coords",
+                   c * len + e);
+            if (a32.is_inf()) partial(1, a32, a29);
+        }
+    }
+}
+
 static fq2_t rand_fq2() { return {rand_fp<FqParams>(), rand_fp<FqParams>()}; }
 
 // a + K*p per component (values below (K + 2) p)
@@ -219,6 +278,7 @@ int main()
     check_field<FqParams>(20000);
     check_field<FrParams>(20000);
     check_madd(200, 80);
+    check_add(150, 60);
     check_fq2(5000);
     check_madd_g2(60, 60);
     if (g_bad) {
