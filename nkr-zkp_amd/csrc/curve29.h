@@ -153,34 +153,38 @@ NKR_HD Xyzz29<P> xyzz29_add(const Xyzz29<P> &a, const Xyzz29<P> &b)
 }
 
 // ---------------------------------------------------------------- G2 (Fq2)
-// Fq2 = Fq[u]/(u^2 + 1) over the 29-bit form.  fp29_mul needs a*b < 2^261 p
-// = 169.28 p^2; a Karatsuba product multiplies the sums of the components,
-// so the bounds are tighter than for G1 and some values are reduced:
-//   fq2_29_mul(a, b), components below A p and B p with A*B <= 40:
-//     components below 5p
-//   fq2_29_sqr(a), components below 5p: c0 below 1.6p, c1 below 2.3p
-//   accumulator  X, Y below 2p; ZZ, ZZZ below 5p; infinity is ZZ = 0 exactly
-//   base         x2, y2 below 8p (fp29_from_mont, then two subtractions)
+// Fq2 = Fq[u]/(u^2 + 1) over the 29-bit form.  Products are schoolbook with
+// each output component one Montgomery reduction of a sum of two (or, for
+// Y3, four) partial products (fp29_mul_add): (a b)_0 = a0 b0 + a1 (Kp - b1),
+// (a b)_1 = a0 b1 + a1 b0 -- the multiply-adds of Karatsuba's three products
+// without its five additions of sums, and with input bounds up to A*B = 84
+// (components below A p and B p; 2AB + 2AK below 2^261/p^2 = 169.28 p^2,
+// K the multiple of p that negates b1), outputs below 2p.  Squares use
+// (a0 + a1)(a0 - a1) and a0 (2 a1), inputs below 6.5p.
+//   accumulator  X below 8p, Y, ZZ, ZZZ below 2p; infinity is ZZ = 0 exactly
+//   base         x2, y2 below 32p (fp29_from_mont, no reduction)
 // madd-2008-s, the sequence of xyzz29_add_affine:
-//   U2 = x2 ZZ, S2 = y2 ZZZ (8p x 5p)                       below 5p
-//   P = U2 + 2p - X, R = S2 + 2p - Y, each reduced by 4p    below 4p
-//   PP = P^2, RR = R^2                                      below 2.3p
-//   PPP = P PP, Q = X PP, ZZ3 = ZZ PP                       below 5p
-//   X3 = (RR + 5p - PPP) + 10p - 2Q, reduced                below 2p
-//   Y3 = R (Q + 2p - X3) + Y (5p - PPP), one reduction per
-//        Karatsuba term (w2: 8p x 14p + 4p x 10p = 152 p^2),
-//        reduced                                            below 2p
-//   ZZZ3 = ZZZ PPP                                          below 5p
+//   U2 = x2 ZZ, S2 = y2 ZZZ (32p x 2p: 2 * 64 = 128)         below 2p
+//   P = U2 + 8p - X, R = S2 + 2p - Y, each reduced by 4p      below 6p
+//   PP = P^2, RR = R^2 ((12p)^2 = 144: c0 < 1.86p; c1 = P0 (2 P1)
+//        72: < 1.43p)                                         below 2p
+//   PPP = P PP, Q = X PP, ZZ3 = ZZ PP (at most 8p x 2p)       below 2p
+//   X3 = (RR + 2p - PPP) + 4p - 2Q                            below 8p
+//   Y3 = R (Q + 8p - X3) + Y (2p - PPP): per component one
+//        reduction of four partial products (6p x 10p twice +
+//        2p x 2p twice = 128)                                 below 2p
+//   ZZZ3 = ZZZ PPP                                            below 2p
 template <class P>
 struct Fq2_29 {
     Fp29<P> c0, c1;
     NKR_HD bool is_exact_zero() const { return c0.is_exact_zero() && c1.is_exact_zero(); }
 };
 
+// canonical -> 29-bit components below 32p (no reduction)
 template <class P>
 NKR_HD Fq2_29<P> fq2_29_from(const fq2_t &a)
 {
-    return {fp29_reduce_to_8p(fp29_from_mont(a.c0)), fp29_reduce_to_8p(fp29_from_mont(a.c1))};
+    return {fp29_from_mont(a.c0), fp29_from_mont(a.c1)};
 }
 
 template <class P>
@@ -189,35 +193,20 @@ NKR_HD fq2_t fq2_29_to(const Fq2_29<P> &a)
     return {fp29_to_mont(a.c0), fp29_to_mont(a.c1)};
 }
 
-template <class P>
+// a * b; components below A p and B p with A*B <= 84 and b1 <= KB p,
+// A*(B + KB) <= 169: components below 2p.
+template <class P, uint32_t KB>
 NKR_HD Fq2_29<P> fq2_29_mul(const Fq2_29<P> &a, const Fq2_29<P> &b)
 {
-    const Fp29<P> v0 = fp29_mul(a.c0, b.c0);
-    const Fp29<P> v1 = fp29_mul(a.c1, b.c1);
-    const Fp29<P> v2 = fp29_mul(fp29_add(a.c0, a.c1), fp29_add(b.c0, b.c1));
-    return {fp29_sub<P, 2>(v0, v1), fp29_sub<P, 3>(v2, fp29_add(v0, v1))};
+    const Fp29<P> nb1 = fp29_sub<P, KB>(Fp29<P>::zero(), b.c1);
+    return {fp29_mul_add(a.c0, b.c0, a.c1, nb1), fp29_mul_add(a.c0, b.c1, a.c1, b.c0)};
 }
 
-// a*b + c*d with one Montgomery reduction per Karatsuba term (fp29_mul_add):
-// w0 = a0 b0 + c0 d0, w1 = a1 b1 + c1 d1, w2 = (a0 + a1)(b0 + b1) +
-// (c0 + c1)(d0 + d1); result (w0 - w1, w2 - w0 - w1).  Needs the w2 inputs
-// below 169 p^2 together (each w below 2p; components: c0 below w0 + 2p,
-// c1 below w2 + 3p).
-template <class P>
-NKR_HD Fq2_29<P> fq2_29_mul_add(const Fq2_29<P> &a, const Fq2_29<P> &b, const Fq2_29<P> &c, const Fq2_29<P> &d)
-{
-    const Fp29<P> w0 = fp29_mul_add(a.c0, b.c0, c.c0, d.c0);
-    const Fp29<P> w1 = fp29_mul_add(a.c1, b.c1, c.c1, d.c1);
-    const Fp29<P> w2 =
-        fp29_mul_add(fp29_add(a.c0, a.c1), fp29_add(b.c0, b.c1), fp29_add(c.c0, c.c1), fp29_add(d.c0, d.c1));
-    return {fp29_sub<P, 2>(w0, w1), fp29_sub<P, 3>(w2, fp29_add(w0, w1))};
-}
-
-template <class P>
+// a^2, components below 6.5p: c0 below 1.86p, c1 below 1.43p.
+template <class P, uint32_t KA>
 NKR_HD Fq2_29<P> fq2_29_sqr(const Fq2_29<P> &a)
 {
-    const Fp29<P> m = fp29_mul(a.c0, a.c1);
-    return {fp29_mul(fp29_add(a.c0, a.c1), fp29_sub<P, 5>(a.c0, a.c1)), fp29_add(m, m)};
+    return {fp29_mul(fp29_add(a.c0, a.c1), fp29_sub<P, KA>(a.c0, a.c1)), fp29_mul(a.c0, fp29_add(a.c1, a.c1))};
 }
 
 template <class P, uint32_t K>
@@ -240,9 +229,9 @@ NKR_HD Fq2_29<P> fq2_29_reduce_to_2p(const Fq2_29<P> &a)
 }
 
 template <class P>
-NKR_HD bool fq2_29_is_zero_below_4p(const Fq2_29<P> &a)
+NKR_HD bool fq2_29_is_zero_below_6p(const Fq2_29<P> &a)
 {
-    if (!(fp29_may_be_zero<P, 4>(a.c0) && fp29_may_be_zero<P, 4>(a.c1))) return false;
+    if (!(fp29_may_be_zero<P, 6>(a.c0) && fp29_may_be_zero<P, 6>(a.c1))) return false;
     return fp29_to_mont(a.c0).is_zero() && fp29_to_mont(a.c1).is_zero();
 }
 
@@ -265,10 +254,10 @@ NKR_HD Xyzz2_29<P> xyzz2_29_from(const Xyzz<fq2_t> &p)
 {
     if (p.ZZ.is_zero()) return Xyzz2_29<P>::infinity();
     Xyzz2_29<P> r;
-    r.X = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.X.c0), fp29_from_mont(p.X.c1)});
-    r.Y = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.Y.c0), fp29_from_mont(p.Y.c1)});
-    r.ZZ = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.ZZ.c0), fp29_from_mont(p.ZZ.c1)});
-    r.ZZZ = fq2_29_reduce_to_2p<P>({fp29_from_mont(p.ZZZ.c0), fp29_from_mont(p.ZZZ.c1)});
+    r.X = fq2_29_reduce_to_2p<P>(fq2_29_from<P>(p.X));
+    r.Y = fq2_29_reduce_to_2p<P>(fq2_29_from<P>(p.Y));
+    r.ZZ = fq2_29_reduce_to_2p<P>(fq2_29_from<P>(p.ZZ));
+    r.ZZZ = fq2_29_reduce_to_2p<P>(fq2_29_from<P>(p.ZZZ));
     return r;
 }
 
@@ -285,40 +274,48 @@ NKR_HD Xyzz<fq2_t> xyzz2_29_to(const Xyzz2_29<P> &p)
 }
 
 // acc + (x2, y2) over G2, complete.  x2, y2: the affine base from
-// fq2_29_from (below 8p).
+// fq2_29_from (below 32p).
 template <class P>
 NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &x2, const Fq2_29<P> &y2)
 {
     if (acc.is_inf()) {
         Xyzz2_29<P> r;
-        r.X = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(x2));
-        r.Y = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(y2));
+        r.X = fq2_29_reduce_to_2p(x2);
+        r.Y = fq2_29_reduce_to_2p(y2);
         r.ZZ.c0 = r.ZZZ.c0 = Fp29<P>::one();
         r.ZZ.c1 = r.ZZZ.c1 = Fp29<P>::zero();
         return r;
     }
-    const Fq2_29<P> Pd = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(fq2_29_mul(x2, acc.ZZ), acc.X));
-    const Fq2_29<P> R = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(fq2_29_mul(y2, acc.ZZZ), acc.Y));
-    if (fq2_29_is_zero_below_4p(Pd)) {
+    const Fq2_29<P> U2 = fq2_29_mul<P, 2>(x2, acc.ZZ);
+    const Fq2_29<P> S2 = fq2_29_mul<P, 2>(y2, acc.ZZZ);
+    Fq2_29<P> Pd = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 8>(U2, acc.X));
+    Fq2_29<P> R = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(S2, acc.Y));
+    if (fq2_29_is_zero_below_6p(Pd)) {
         // the base is +-(the accumulator): the canonical formulas decide
-        if (fq2_29_is_zero_below_4p(R))
+        if (fq2_29_is_zero_below_6p(R))
             return xyzz2_29_from<P>(xyzz_dbl_affine(fq2_29_to(x2), fq2_29_to(y2)));
         return Xyzz2_29<P>::infinity();
     }
-    const Fq2_29<P> PP = fq2_29_sqr(Pd);
-    const Fq2_29<P> PPP = fq2_29_mul(Pd, PP);
-    const Fq2_29<P> RR = fq2_29_sqr(R);
-    const Fq2_29<P> Q = fq2_29_mul(acc.X, PP);
+    fp29_opaque(Pd.c0);
+    fp29_opaque(Pd.c1);
+    fp29_opaque(R.c0);
+    fp29_opaque(R.c1);
+    const Fq2_29<P> PP = fq2_29_sqr<P, 6>(Pd);
+    const Fq2_29<P> PPP = fq2_29_mul<P, 2>(Pd, PP);
+    const Fq2_29<P> RR = fq2_29_sqr<P, 6>(R);
+    const Fq2_29<P> Q = fq2_29_mul<P, 2>(acc.X, PP);
     Xyzz2_29<P> r;
-    r.ZZ = fq2_29_mul(acc.ZZ, PP);
+    r.ZZ = fq2_29_mul<P, 2>(acc.ZZ, PP);
     const Fq2_29<P> Q2 = {fp29_add(Q.c0, Q.c0), fp29_add(Q.c1, Q.c1)};
-    r.X = fq2_29_sub_if_ge<P, 2>(fq2_29_sub_if_ge<P, 4>(fq2_29_sub_if_ge<P, 8>(
-        fq2_29_sub_if_ge<P, 16>(fq2_29_sub<P, 10>(fq2_29_sub<P, 5>(RR, PPP), Q2)))));
-    const Fq2_29<P> T = fq2_29_sub<P, 2>(Q, r.X);
-    const Fq2_29<P> N = fq2_29_sub<P, 5>({Fp29<P>::zero(), Fp29<P>::zero()}, PPP);
-    r.ZZZ = fq2_29_mul(acc.ZZZ, PPP);
-    const Fq2_29<P> y = fq2_29_mul_add(R, T, acc.Y, N);  // c0 below 3.3p, c1 below 5p
-    r.Y = {fp29_sub_if_ge<P, 2>(y.c0), fp29_sub_if_ge<P, 2>(fp29_sub_if_ge<P, 4>(y.c1))};
+    r.X = fq2_29_sub<P, 4>(fq2_29_sub<P, 2>(RR, PPP), Q2);
+    const Fq2_29<P> T = fq2_29_sub<P, 8>(Q, r.X);
+    const Fq2_29<P> N = fq2_29_sub<P, 2>({Fp29<P>::zero(), Fp29<P>::zero()}, PPP);
+    r.ZZZ = fq2_29_mul<P, 2>(acc.ZZZ, PPP);
+    // Y3 = R T + Y N, per component one reduction of four partial products
+    const Fp29<P> nT1 = fp29_sub<P, 10>(Fp29<P>::zero(), T.c1);
+    const Fp29<P> nN1 = fp29_sub<P, 2>(Fp29<P>::zero(), N.c1);
+    r.Y.c0 = fp29_mul_add4(R.c0, T.c0, R.c1, nT1, acc.Y.c0, N.c0, acc.Y.c1, nN1);
+    r.Y.c1 = fp29_mul_add4(R.c0, T.c1, R.c1, T.c0, acc.Y.c0, N.c1, acc.Y.c1, N.c0);
     return r;
 }
 

@@ -450,6 +450,45 @@ NKR_HD Fp29<P> fp29_mul_add(const Fp29<P> &a, const Fp29<P> &b, const Fp29<P> &c
     return r;
 }
 
+// (a*b + c*d + e*f + g*h)*2^-261 mod p with one Montgomery reduction (36
+// product terms and 9 reduction terms per column, below 45 * 2^58 < 2^64).
+// Inputs: limbs below 2^29, the sum of the four products below 2^261 * p.
+// Output: limbs below 2^29, value below p + sum/2^261 < 2p.
+template <class P>
+NKR_HD Fp29<P> fp29_mul_add4(const Fp29<P> &a, const Fp29<P> &b, const Fp29<P> &c, const Fp29<P> &d,
+                             const Fp29<P> &e, const Fp29<P> &f, const Fp29<P> &g, const Fp29<P> &h)
+{
+    constexpr uint32_t INV = P29<P>::inv();
+    uint32_t m[9];
+    Fp29<P> r;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+#pragma unroll
+        for (int i = 0; i <= k; ++i)
+            acc += (uint64_t)a.l[i] * b.l[k - i] + (uint64_t)c.l[i] * d.l[k - i] + (uint64_t)e.l[i] * f.l[k - i] +
+                   (uint64_t)g.l[i] * h.l[k - i];
+#pragma unroll
+        for (int i = 0; i < k; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        m[k] = ((uint32_t)acc * INV) & kMask29;
+        acc += (uint64_t)m[k] * P29<P>::bits(0);
+        acc >>= 29;
+    }
+#pragma unroll
+    for (int k = 9; k < 17; ++k) {
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i)
+            acc += (uint64_t)a.l[i] * b.l[k - i] + (uint64_t)c.l[i] * d.l[k - i] + (uint64_t)e.l[i] * f.l[k - i] +
+                   (uint64_t)g.l[i] * h.l[k - i];
+#pragma unroll
+        for (int i = k - 8; i < 9; ++i) acc += (uint64_t)m[i] * P29<P>::bits(k - i);
+        r.l[k - 9] = (uint32_t)acc & kMask29;
+        acc >>= 29;
+    }
+    r.l[8] = (uint32_t)acc;
+    return r;
+}
+
 // Two independent products r0 = a0*b0 and r1 = a1*b1 (fp29_mul's bounds for
 // each) with their multiply-adds interleaved: each product's column sum is
 // one dependent chain through its accumulator, so a wave that alternates two

@@ -15,8 +15,9 @@
 //   * full addition: chains of xyzz29_add over partial sums (ZZ != 1),
 //     doublings, cancellations and infinity operands give xyzz_add's
 //     coordinates after every step;
-//   * the same over Fq2 (G2): Karatsuba products and squares at their input
-//     bounds, and chains of xyzz2_29_add_affine against xyzz_add_affine.
+//   * the same over Fq2 (G2): products, squares and the four-product sum at
+//     their input bounds, and chains of xyzz2_29_add_affine against
+//     xyzz_add_affine.
 //     The formulas are field identities, so random coordinates (not on the
 //     curve) exercise them as well as curve points do.
 // Prints "ok <checks>" or the first mismatches; exit status 1 on a mismatch.
@@ -227,16 +228,26 @@ static void check_fq2(int n)
     for (int i = 0; i < n; ++i) {
         const fq2_t a = rand_fq2(), b = rand_fq2();
         const fq2_t want = fq2_mul(a, b);
-        expect(fq2_29_to(fq2_29_mul(widen2<6>(a), widen2<3>(b))) == want, "fq2 mul 8p x 5p", i);
-        expect(fq2_29_to(fq2_29_mul(widen2<3>(a), widen2<6>(b))) == want, "fq2 mul 5p x 8p", i);
-        expect(fq2_29_to(fq2_29_sqr(widen2<3>(a))) == fq2_sqr(a), "fq2 sqr 5p", i);
+        // the products' input bounds in the G2 addition: 32p x 2p (U2, S2),
+        // 8p x 2p (Q), 6p x 2p (PPP), 2p x 2p (ZZ3, ZZZ3)
+        expect(fq2_29_to(fq2_29_mul<FqParams, 2>(widen2<30>(a), widen2<0>(b))) == want, "fq2 mul 32p x 2p", i);
+        expect(fq2_29_to(fq2_29_mul<FqParams, 2>(widen2<6>(a), widen2<0>(b))) == want, "fq2 mul 8p x 2p", i);
+        expect(fq2_29_to(fq2_29_mul<FqParams, 2>(widen2<0>(a), widen2<0>(b))) == want, "fq2 mul 2p x 2p", i);
+        expect(fq2_29_to(fq2_29_sqr<FqParams, 6>(widen2<4>(a))) == fq2_sqr(a), "fq2 sqr 6p", i);
         {
-            // the G2 Y3 shape: components 4p x 7p + 2p x 5p
+            // Y3's shape: R (6p) T (9.2p, as 10p) + Y (2p) N (2p), per component
             const fq2_t c = rand_fq2(), d = rand_fq2();
-            const Fq2_29<FqParams> y = fq2_29_mul_add(widen2<2>(a), widen2<5>(b), widen2<0>(c), widen2<3>(d));
-            expect(fq2_29_to(y) == fq2_add(fq2_mul(a, b), fq2_mul(c, d)), "fq2 mul_add", i);
+            const Fq2_29<FqParams> R = widen2<4>(a), T = widen2<8>(b), Y = widen2<0>(c), N = widen2<0>(d);
+            const Fp29<FqParams> nT1 = fp29_sub<FqParams, 10>(Fp29<FqParams>::zero(), T.c1);
+            const Fp29<FqParams> nN1 = fp29_sub<FqParams, 2>(Fp29<FqParams>::zero(), N.c1);
+            const Fq2_29<FqParams> y = {fp29_mul_add4(R.c0, T.c0, R.c1, nT1, Y.c0, N.c0, Y.c1, nN1),
+                                        fp29_mul_add4(R.c0, T.c1, R.c1, T.c0, Y.c0, N.c1, Y.c1, N.c0)};
+            expect(fq2_29_to(y) == fq2_add(fq2_mul(a, b), fq2_mul(c, d)), "fq2 four-product sum", i);
+            bool limbs = true;
+            for (int k = 0; k < 9; ++k) limbs = limbs && (y.c0.l[k] >> 29) == 0 && (y.c1.l[k] >> 29) == 0;
+            expect(limbs, "fq2 four-product limbs", i);
         }
-        const Fq2_29<FqParams> m = fq2_29_mul(widen2<6>(a), widen2<3>(b));
+        const Fq2_29<FqParams> m = fq2_29_mul<FqParams, 2>(widen2<30>(a), widen2<0>(b));
         bool limbs = true;
         for (int k = 0; k < 9; ++k) limbs = limbs && (m.c0.l[k] >> 29) == 0 && (m.c1.l[k] >> 29) == 0;
         expect(limbs, "fq2 product limbs", i);
