@@ -273,6 +273,54 @@ NKR_HD Xyzz<fq2_t> xyzz2_29_to(const Xyzz2_29<P> &p)
     return r;
 }
 
+// a + b over G2, complete (add-2008-s), both in the accumulator invariant,
+// for the bucket reduction (the G2 partial sums stay in this form):
+//   U1 = X1 ZZ2, U2 = X2 ZZ1 (8p x 2p), S1 = Y1 ZZZ2, S2 = Y2 ZZZ1  below 2p
+//   P = U2 + 2p - U1, R = S2 + 2p - S1                          below 4p
+//   PP = P^2, RR = R^2, PPP = P PP, Q = U1 PP                    below 2p
+//   X3 = (RR + 2p - PPP) + 4p - 2Q                               below 8p
+//   Y3 = R (Q + 8p - X3) + S1 (2p - PPP): per component one
+//        reduction of four partial products (4p x 10p twice +
+//        2p x 2p twice = 88)                                     below 2p
+//   ZZ3 = (ZZ1 ZZ2) PP, ZZZ3 = (ZZZ1 ZZZ2) PPP                   below 2p
+// P = 0 (b = +-a) takes the canonical doubling or gives infinity.
+template <class P>
+NKR_HD Xyzz2_29<P> xyzz2_29_add(const Xyzz2_29<P> &a, const Xyzz2_29<P> &b)
+{
+    if (a.is_inf()) return b;
+    if (b.is_inf()) return a;
+    const Fq2_29<P> U1 = fq2_29_mul<P, 2>(a.X, b.ZZ);
+    const Fq2_29<P> U2 = fq2_29_mul<P, 2>(b.X, a.ZZ);
+    const Fq2_29<P> S1 = fq2_29_mul<P, 2>(a.Y, b.ZZZ);
+    const Fq2_29<P> S2 = fq2_29_mul<P, 2>(b.Y, a.ZZZ);
+    Fq2_29<P> Pd = fq2_29_sub<P, 2>(U2, U1);
+    Fq2_29<P> R = fq2_29_sub<P, 2>(S2, S1);
+    if (fq2_29_is_zero_below_6p(Pd)) {
+        if (fq2_29_is_zero_below_6p(R)) return xyzz2_29_from<P>(xyzz_dbl(xyzz2_29_to(a)));
+        return Xyzz2_29<P>::infinity();
+    }
+    fp29_opaque(Pd.c0);
+    fp29_opaque(Pd.c1);
+    fp29_opaque(R.c0);
+    fp29_opaque(R.c1);
+    const Fq2_29<P> PP = fq2_29_sqr<P, 4>(Pd);
+    const Fq2_29<P> PPP = fq2_29_mul<P, 2>(Pd, PP);
+    const Fq2_29<P> Q = fq2_29_mul<P, 2>(U1, PP);
+    const Fq2_29<P> RR = fq2_29_sqr<P, 4>(R);
+    Xyzz2_29<P> r;
+    r.ZZ = fq2_29_mul<P, 2>(fq2_29_mul<P, 2>(a.ZZ, b.ZZ), PP);
+    r.ZZZ = fq2_29_mul<P, 2>(fq2_29_mul<P, 2>(a.ZZZ, b.ZZZ), PPP);
+    const Fq2_29<P> Q2 = {fp29_add(Q.c0, Q.c0), fp29_add(Q.c1, Q.c1)};
+    r.X = fq2_29_sub<P, 4>(fq2_29_sub<P, 2>(RR, PPP), Q2);
+    const Fq2_29<P> T = fq2_29_sub<P, 8>(Q, r.X);
+    const Fq2_29<P> N = fq2_29_sub<P, 2>({Fp29<P>::zero(), Fp29<P>::zero()}, PPP);
+    const Fp29<P> nT1 = fp29_sub<P, 10>(Fp29<P>::zero(), T.c1);
+    const Fp29<P> nN1 = fp29_sub<P, 2>(Fp29<P>::zero(), N.c1);
+    r.Y.c0 = fp29_mul_add4(R.c0, T.c0, R.c1, nT1, S1.c0, N.c0, S1.c1, nN1);
+    r.Y.c1 = fp29_mul_add4(R.c0, T.c1, R.c1, T.c0, S1.c0, N.c1, S1.c1, N.c0);
+    return r;
+}
+
 // acc + (x2, y2) over G2, complete.  x2, y2: the affine base from
 // fq2_29_from (below 32p).
 template <class P>

@@ -180,16 +180,17 @@ private:
 };
 
 // The form of the partial sums between the accumulation and the bit planes:
-// G1 keeps the reduced-radix form (curve29.h) through fold, completion and
-// tree, so no addition converts; G2 partial sums are canonical XYZZ.  The
-// bit planes handed to the host are canonical either way.
+// the reduced-radix form (curve29.h) through fold, completion and tree, so
+// no addition converts.  The bit planes handed to the host are canonical.
 template <class F>
-struct MsmPartial {
-    using type = Xyzz<F>;
-};
+struct MsmPartial;
 template <>
 struct MsmPartial<fq_t> {
     using type = Xyzz29<FqParams>;
+};
+template <>
+struct MsmPartial<fq2_t> {
+    using type = Xyzz2_29<FqParams>;
 };
 template <class F>
 using partial_t = typename MsmPartial<F>::type;

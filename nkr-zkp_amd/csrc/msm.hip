@@ -57,12 +57,12 @@ __device__ __forceinline__ Xyzz29<FqParams> partial_add(const Xyzz29<FqParams> &
 {
     return xyzz29_add(a, b);
 }
-__device__ __forceinline__ Xyzz<fq2_t> partial_add(const Xyzz<fq2_t> &a, const Xyzz<fq2_t> &b)
+__device__ __forceinline__ Xyzz2_29<FqParams> partial_add(const Xyzz2_29<FqParams> &a, const Xyzz2_29<FqParams> &b)
 {
-    return xyzz_add(a, b);
+    return xyzz2_29_add(a, b);
 }
 __device__ __forceinline__ Xyzz<fq_t> partial_plane(const Xyzz29<FqParams> &a) { return xyzz29_to(a); }
-__device__ __forceinline__ Xyzz<fq2_t> partial_plane(const Xyzz<fq2_t> &a) { return a; }
+__device__ __forceinline__ Xyzz<fq2_t> partial_plane(const Xyzz2_29<FqParams> &a) { return xyzz2_29_to(a); }
 
 // ------------------------------------------------------------ digit recoding
 // Signed c-bit digits of an integer scalar (< 2^254): d_w in [-2^(c-1), 2^(c-1)],
@@ -445,7 +445,7 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
         partial_t<F> acc = partial_t<F>::infinity();
         if constexpr (std::is_same<F, fq2_t>::value) {
             // G2: the accumulator in the reduced-radix form as for G1
-            // (curve29.h, Fq2 bounds); no prefetch.
+            // (curve29.h, Fq2 bounds), stored in it; no prefetch.
             Xyzz2_29<FqParams> acc29 = Xyzz2_29<FqParams>::infinity();
 #pragma unroll 1
             for (uint32_t e = beg; e < end; ++e) {
@@ -456,7 +456,7 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
                     acc29 = xyzz2_29_add_affine(acc29, fq2_29_from<FqParams>(P.x), fq2_29_from<FqParams>(y));
                 }
             }
-            acc = xyzz2_29_to(acc29);
+            acc = acc29;
         } else if (beg < end) {
             // G1: the accumulator stays in the reduced-radix form
             // (curve29.h: one v_mad_u64_u32 per product term, no carries)

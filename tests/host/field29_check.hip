@@ -284,6 +284,50 @@ static void check_madd_g2(int chains, int len)
     }
 }
 
+// Full G2 additions over partial sums, as check_add.
+static void check_add_g2(int chains, int len)
+{
+    std::vector<Affine<fq2_t>> pool;
+    for (int i = 0; i < 16; ++i) pool.push_back({rand_fq2(), rand_fq2()});
+    auto partial = [&](int n, Xyzz<fq2_t> &c32, Xyzz2_29<FqParams> &c29) {
+        c32 = Xyzz<fq2_t>::infinity();
+        c29 = Xyzz2_29<FqParams>::infinity();
+        for (int e = 0; e < n; ++e) {
+            const Affine<fq2_t> b = pool[rnd() % pool.size()];
+            c32 = xyzz_add_affine(c32, b.x, b.y);
+            c29 = xyzz2_29_add_affine(c29, fq2_29_from<FqParams>(b.x), fq2_29_from<FqParams>(b.y));
+        }
+    };
+    for (int c = 0; c < chains; ++c) {
+        Xyzz<fq2_t> a32;
+        Xyzz2_29<FqParams> a29;
+        partial(1 + rnd() % 3, a32, a29);
+        for (int e = 0; e < len; ++e) {
+            const uint32_t r = rnd();
+            Xyzz<fq2_t> b32;
+            Xyzz2_29<FqParams> b29;
+            partial((r >> 4) % 4, b32, b29);
+            if ((r >> 8) % 6 == 0) {
+                b32 = a32;
+                b29 = a29;
+                ++g_doublings;
+            } else if ((r >> 12) % 7 == 0 && !a32.is_inf()) {
+                b32 = a32;
+                b32.Y = fq2_neg(b32.Y);
+                b29 = a29;
+                b29.Y = fq2_29_sub<FqParams, 2>({Fp29<FqParams>::zero(), Fp29<FqParams>::zero()}, a29.Y);
+                ++g_cancels;
+            }
+            a32 = xyzz_add(a32, b32);
+            a29 = xyzz2_29_add(a29, b29);
+            const Xyzz<fq2_t> got = xyzz2_29_to(a29);
+            expect(got.X == a32.X && got.Y == a32.Y && got.ZZ == a32.ZZ && got.ZZZ == a32.ZZZ, "g2 add chain",
+                   c * len + e);
+            if (a32.is_inf()) partial(1, a32, a29);
+        }
+    }
+}
+
 int main()
 {
     check_field<FqParams>(20000);
@@ -292,6 +336,7 @@ int main()
     check_add(150, 60);
     check_fq2(5000);
     check_madd_g2(60, 60);
+    check_add_g2(60, 40);
     if (g_bad) {
         std::printf("FAILED %d of %ld checks\n", g_bad, g_checks);
         return 1;
