@@ -211,6 +211,20 @@ def test_msm_duplicate_and_opposite_bases():
     assert np.array_equal(got, O.g1_to_affine(O.msm_g1(bases, s))[0])
 
 
+def test_msm_g2_duplicate_and_opposite_bases():
+    """The same over G2: P + P and P + (-P) inside the bucket pieces and
+    between their partial sums (the reduced-radix full addition's doubling
+    and cancellation)."""
+    g2 = O.g2_generator()
+    base = N.fixed_base_mul(g2, rand_fr(811, 1), g2=True)[0]
+    minus_one = np.array(O.int_to_words(O.to_mont_int(O.R - 1, O.R)), dtype=np.uint64).reshape(1, 4)
+    neg = O.g2_to_affine(O.g2_mul(base, minus_one))[0]
+    bases = np.stack([base] * 40 + [neg] * 7)
+    s = np.repeat(rand_fr(812, 1), 47, axis=0)
+    got = N.msm(bases, s, g2=True)
+    assert np.array_equal(got, O.g2_to_affine(O.msm_g2(bases, s))[0])
+
+
 def test_msm_g2_vs_oracle():
     n = 2048
     g2 = O.g2_generator()
