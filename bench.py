@@ -536,7 +536,9 @@ def batch_leg(shapes, items, per_shape: int, reps: int) -> dict:
     """Batched vs plain proofs/s over the heterogeneous mix (16 circuit shapes,
     domains 2^9 .. 2^18, per_shape witnesses each).  Plain: one
     nkr_g16_prove_blind per proof, back to back.  Batched: the same proofs in
-    one nkr_g16_batch_prove call.  Both time host-resident witnesses in and
+    one nkr_g16_batch_prove call, the repeated composition replayed as a
+    captured HIP graph (the eager launch sequence timed beside it).  Both
+    time host-resident witnesses in and
     blinded proofs out; every batched proof must equal its plain proof bit for
     bit and verify."""
     plain_out = [pv.prove_blind(w, r, s) for (_, _, pv, _, w, r, s) in items]  # warm-up + reference results
@@ -548,20 +550,35 @@ def batch_leg(shapes, items, per_shape: int, reps: int) -> dict:
 
     batch = N.Batch()
     args = [(pv, w, r, s) for (_, _, pv, _, w, r, s) in items]
-    outs = batch.prove(args)  # warm-up (work buffers grow here)
+    # eager launch sequence (graph capture off)
+    batch.set_graphs(False)
+    outs_e = batch.prove(args)  # warm-up (work buffers grow here)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        outs_e = batch.prove(args)
+    eager_s = (time.perf_counter() - t0) / reps
+    # the repeated composition as a captured HIP graph: one eager call records
+    # the composition, the next captures and replays, the timed calls replay
+    batch.set_graphs(True)
+    batch.prove(args)
+    batch.prove(args)
+    graph = batch.graph_active()
     t0 = time.perf_counter()
     for _ in range(reps):
         outs = batch.prove(args)
     batch_s = (time.perf_counter() - t0) / reps
     exact = all(np.array_equal(o[0], p[0]) and np.array_equal(o[1], p[1]) for o, p in zip(outs, plain_out))
+    exact = exact and all(np.array_equal(o[0], p[0]) and np.array_equal(o[1], p[1])
+                          for o, p in zip(outs_e, plain_out))
     verified = all(N.verify(keys, w[1:1 + circ.public_inputs], o[1])
                    for (_, circ, _, keys, w, _, _), o in zip(items, outs))
     batch.close()
     n = len(items)
-    log(f"batch mix: plain {1000 * plain_s:.1f} ms, batched {1000 * batch_s:.1f} ms for {n} proofs; "
-        f"bit-exact={exact} verified={verified}")
+    log(f"batch mix: plain {1000 * plain_s:.1f} ms, batched {1000 * batch_s:.1f} ms for {n} proofs "
+        f"(graph replay {graph}; eager launches {1000 * eager_s:.1f} ms); bit-exact={exact} verified={verified}")
     return {"proofs": n, "shapes": shapes, "per_shape": per_shape,
             "plain_ms": round(1000 * plain_s, 3), "batched_ms": round(1000 * batch_s, 3),
+            "batched_graph_replay": graph, "batched_eager_ms": round(1000 * eager_s, 3),
             "plain_proofs_per_s": round(n / plain_s, 2), "batched_proofs_per_s": round(n / batch_s, 2),
             "speedup": round(plain_s / batch_s, 3), "bit_exact_vs_plain": exact, "all_verified": verified,
             "reps": reps}

@@ -152,6 +152,16 @@ typedef struct {
 int nkr_g16_batch_create(int device, nkr_g16_batch **out);
 int nkr_g16_batch_prove(nkr_g16_batch *batch, const nkr_g16_batch_item *items, uint64_t count);
 int nkr_g16_batch_destroy(nkr_g16_batch *batch);
+/* Launch-sequence reuse (no counterpart in the reference, which launches
+ * every kernel of every proof): a batch of the same contexts in the same
+ * order as the handle's previous call is captured into a HIP graph on its
+ * second occurrence and replayed while the composition repeats (the
+ * size-class schedule of that composition); any other batch runs eagerly and
+ * drops the graph.  Results are identical either way.  enabled = 0 turns the
+ * capture off (and drops a graph); *active tells whether the handle holds a
+ * graph now. */
+int nkr_g16_batch_set_graphs(nkr_g16_batch *batch, int enabled);
+int nkr_g16_batch_graph_active(const nkr_g16_batch *batch, int *active);
 
 /* Serving: a request loop with continuous batching (no counterpart in the
  * reference, which serves nothing; its harness runs one prove per process,

@@ -24,6 +24,12 @@
 //               copy as soon as it is staged.
 //   bit planes  three batch-wide regions (G1 over the witness, H, G2), one
 //               device-to-host copy each.
+//   graphs      a batch of the same contexts in the same order as the
+//               previous call is captured into a HIP graph (one per prover)
+//               and replayed while the composition repeats: the fold levels
+//               are then launched for the capacity's worst case and nothing
+//               is read back mid-sequence.  Any other batch runs eagerly and
+//               drops the graph.
 //   host tail   the r,s-only blinding terms while the GPU works, then the
 //               final sums and blinding on a pool of host threads.
 // Each proof's result is bit-identical to proving it alone.
@@ -65,9 +71,14 @@ public:
     // prove reallocates what it needs.
     uint64_t device_bytes() const;
     void trim();
+    // Whether a batch repeating the previous call's contexts (same order) is
+    // captured into a HIP graph and replayed (default on).
+    void set_graphs(bool enabled);
+    bool graph_active() const { return graph_exec_ != nullptr; }
 
 private:
     void release() noexcept;  // streams and events (destructor, failed constructor)
+    void drop_graph() noexcept;
     struct Slot {
         DeviceBuffer<fr_t> z, pa, pb, pc;
         MsmSchedule wsched, hsched;
@@ -77,6 +88,11 @@ private:
     hipStream_t stream2_ = nullptr;  // witness schedules and G2 MSMs, beside them
     hipEvent_t ev_z_ = nullptr, ev_wsched_ = nullptr, ev_side_ = nullptr;
     std::vector<std::unique_ptr<Slot>> slots_;
+    // the captured launch sequence, the contexts it was captured for, and
+    // the previous call's contexts
+    hipGraphExec_t graph_exec_ = nullptr;
+    std::vector<const void *> graph_key_, last_key_;
+    bool graphs_enabled_ = true;
     MsmScheduleBatch sort_w_, sort_h_;
     MsmBatch<fq_t> batch_g1_;
     MsmBatch<fq2_t> batch_g2_;

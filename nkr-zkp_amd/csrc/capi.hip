@@ -297,6 +297,22 @@ int nkr_g16_batch_destroy(nkr_g16_batch *batch)
     return guarded([&] { delete batch; });
 }
 
+int nkr_g16_batch_set_graphs(nkr_g16_batch *batch, int enabled)
+{
+    return guarded([&] {
+        NKR_REQUIRE(batch, "null argument");
+        batch->prover->set_graphs(enabled != 0);
+    });
+}
+
+int nkr_g16_batch_graph_active(const nkr_g16_batch *batch, int *active)
+{
+    return guarded([&] {
+        NKR_REQUIRE(batch && active, "null argument");
+        *active = batch->prover->graph_active() ? 1 : 0;
+    });
+}
+
 int nkr_g16_server_create(int device, uint32_t workers, uint32_t max_batch, uint64_t work_budget_bytes,
                           nkr_g16_server **out)
 {

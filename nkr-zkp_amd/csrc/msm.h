@@ -164,7 +164,12 @@ public:
         const fr_t *scalars_mont;
         MsmPlan plan;
     };
-    void build(const std::vector<Item> &items, hipStream_t stream);
+    // graph: the launches are being captured into a graph that is replayed
+    // later, so nothing here may wait for or read device results on the host
+    // (the schedules' counters are not copied back and no staging event is
+    // recorded; the caller synchronises its streams before the staging
+    // buffers are written again).
+    void build(const std::vector<Item> &items, hipStream_t stream, bool graph = false);
     uint64_t device_bytes() const { return table_.bytes() + totals_dev_.bytes(); }
     void release_device()
     {
@@ -210,7 +215,12 @@ struct MsmJob {
 template <class F>
 class MsmBatch {
 public:
-    void run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer = nullptr);
+    // graph: as MsmScheduleBatch::build; the fold levels are then launched
+    // for the largest bucket the partial-sum capacity allows (levels a batch
+    // does not need find nothing to fold) instead of for the counters read
+    // back from the schedules.
+    void run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer = nullptr,
+             bool graph = false);
     // Device bytes held by the batch's work buffers, and their release (the
     // next run reallocates what it needs).
     uint64_t device_bytes() const
@@ -236,6 +246,9 @@ public:
         fold_start_.release();
         key_.clear();  // the layout cache refers to the released buffers
     }
+
+    // The jobs' layout is the one the batch's buffers already hold.
+    bool layout_matches(const std::vector<MsmJob<F>> &jobs) const;
 
 private:
     void layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream);

@@ -875,14 +875,14 @@ void MsmSchedule::build(const fr_t *scalars_mont, const MsmPlan &plan, hipStream
     if (timer) timer->end(kStageSort, stream);
 }
 
-void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
+void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream, bool graph)
 {
     const uint32_t J = (uint32_t)items.size();
     if (J == 0) return;
     NKR_REQUIRE(J <= 65535, "a schedule batch holds at most 65535 schedules");
     const size_t jobs_bytes = (sizeof(SortJob) * J + 15) & ~size_t(15);
     const size_t total = jobs_bytes + sizeof(uint32_t) * (J + 1);
-    uint8_t *st = stage_.reuse(total);
+    uint8_t *st = graph ? stage_.reuse_unsynced(total) : stage_.reuse(total);
     table_.ensure(total);
     SortJob *jobs = reinterpret_cast<SortJob *>(st);
     uint32_t *off = reinterpret_cast<uint32_t *>(st + jobs_bytes);
@@ -904,7 +904,7 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     NKR_REQUIRE(blocks < (1ull << 31), "schedule batch too large");
     off[J] = (uint32_t)blocks;
     NKR_HIP(hipMemcpyAsync(table_.get(), st, total, hipMemcpyHostToDevice, stream));
-    stage_.uploaded(stream);
+    if (!graph) stage_.uploaded(stream);
     const SortJob *dj = reinterpret_cast<const SortJob *>(table_.get());
     const uint32_t *doff = reinterpret_cast<const uint32_t *>(table_.get() + jobs_bytes);
     const size_t lds = (size_t)Bmax * sizeof(uint32_t);
@@ -914,16 +914,23 @@ void MsmScheduleBatch::build(const std::vector<Item> &items, hipStream_t stream)
     NKR_LAUNCH_CHECK();
     k_schedule_scan_batch<<<J, 1024, 0, stream>>>(dj);
     NKR_LAUNCH_CHECK();
-    // every schedule's four counters to the host in one copy
-    totals_dev_.ensure(4 * J);
-    k_gather_totals<<<div_up(4 * J, 256), 256, 0, stream>>>(dj, J, totals_dev_.get());
-    NKR_LAUNCH_CHECK();
-    uint32_t *th = totals_host_.reuse(4 * J);
-    NKR_HIP(hipMemcpyAsync(th, totals_dev_.get(), 4 * J * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    totals_host_.uploaded(stream);
-    for (uint32_t j = 0; j < J; ++j) {
-        items[j].sched->host_totals_ = th + 4 * j;
-        items[j].sched->host_stage_ = &totals_host_;
+    if (!graph) {
+        // every schedule's four counters to the host in one copy
+        totals_dev_.ensure(4 * J);
+        k_gather_totals<<<div_up(4 * J, 256), 256, 0, stream>>>(dj, J, totals_dev_.get());
+        NKR_LAUNCH_CHECK();
+        uint32_t *th = totals_host_.reuse(4 * J);
+        NKR_HIP(hipMemcpyAsync(th, totals_dev_.get(), 4 * J * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+        totals_host_.uploaded(stream);
+        for (uint32_t j = 0; j < J; ++j) {
+            items[j].sched->host_totals_ = th + 4 * j;
+            items[j].sched->host_stage_ = &totals_host_;
+        }
+    } else {
+        for (uint32_t j = 0; j < J; ++j) {
+            items[j].sched->host_totals_ = nullptr;
+            items[j].sched->host_stage_ = nullptr;
+        }
     }
     k_scatter_batch<<<(uint32_t)blocks, threads, lds, stream>>>(dj, doff, J);
     NKR_LAUNCH_CHECK();
@@ -947,7 +954,7 @@ static uint64_t job_max_pieces(const MsmJob<F> &jb)
 }
 
 template <class F>
-void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
+static std::vector<const void *> layout_key(const std::vector<MsmJob<F>> &jobs)
 {
     std::vector<const void *> key;
     for (const auto &jb : jobs) {
@@ -958,6 +965,19 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
         key.push_back(jb.planes);
         key.push_back(reinterpret_cast<const void *>((uintptr_t)job_max_pieces(jb)));
     }
+    return key;
+}
+
+template <class F>
+bool MsmBatch<F>::layout_matches(const std::vector<MsmJob<F>> &jobs) const
+{
+    return layout_key(jobs) == key_;
+}
+
+template <class F>
+void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
+{
+    std::vector<const void *> key = layout_key(jobs);
     if (key == key_) return;
     const uint32_t J = (uint32_t)jobs.size();
     // Per job, sized by its own plan: the partial-sum pool (level-0 pieces,
@@ -1036,9 +1056,10 @@ void MsmBatch<F>::layout(const std::vector<MsmJob<F>> &jobs, hipStream_t stream)
 }
 
 template <class F>
-void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer)
+void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, StageTimer *timer, bool graph)
 {
     if (jobs.empty()) return;
+    NKR_REQUIRE(!graph || layout_matches(jobs), "a captured MSM batch must reuse the layout of its eager run");
     layout(jobs, stream);
     const uint32_t J = (uint32_t)jobs.size();
     uint32_t Bmax = 0, cmax = 0;
@@ -1066,9 +1087,13 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     // copy of the schedules' counters.  The accumulation is already queued,
     // so this wait overlaps the GPU's work instead of stalling it.
     uint32_t folds = 0;
-    for (const auto &jb : jobs) {
-        const uint64_t cnt = (jb.alt ? jb.sched->alt_totals_host() : jb.sched->totals_host())[2];
-        folds = std::max(folds, msm_fold_levels(cnt));
+    if (graph) {
+        folds = fold_levels_;
+    } else {
+        for (const auto &jb : jobs) {
+            const uint64_t cnt = (jb.alt ? jb.sched->alt_totals_host() : jb.sched->totals_host())[2];
+            folds = std::max(folds, msm_fold_levels(cnt));
+        }
     }
     NKR_REQUIRE(folds <= fold_levels_, "MSM bucket exceeds the partial-sum capacity");
     for (uint32_t L = 0; L < folds; ++L) {

@@ -113,6 +113,14 @@ public:
     T *reuse(size_t count)
     {
         if (event_) NKR_HIP(hipEventSynchronize(event_));
+        return reuse_unsynced(count);
+    }
+    // reuse() without the wait, for a caller that knows the buffer's
+    // transfers are done -- e.g. while capturing a graph after synchronising
+    // its streams, where waiting on an event last recorded on a stream that
+    // is now capturing is not permitted.
+    T *reuse_unsynced(size_t count)
+    {
         if (count > count_) {
             if (ptr_) NKR_HIP(hipHostFree(ptr_));
             ptr_ = nullptr;

@@ -131,6 +131,8 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_batch_create": [i32, ctypes.POINTER(vp)],
             "nkr_g16_batch_prove": [vp, ctypes.POINTER(BatchItem), u64],
             "nkr_g16_batch_destroy": [vp],
+            "nkr_g16_batch_set_graphs": [vp, i32],
+            "nkr_g16_batch_graph_active": [vp, ctypes.POINTER(i32)],
             "nkr_g16_server_create": [i32, u32, u32, u64, ctypes.POINTER(vp)],
             "nkr_g16_server_submit": [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64)],
             "nkr_g16_server_wait": [vp, u64, ctypes.POINTER(ctypes.c_double)],
@@ -388,6 +390,15 @@ class Batch:
             arr[i] = BatchItem(pv.h.value, _p(w), _p(r), _p(s), _p(raw), _p(proof))
         _check(lib().nkr_g16_batch_prove(self.h, arr, len(items)))
         return outs
+
+    def set_graphs(self, enabled: bool) -> None:
+        """Capture a repeated composition into a HIP graph and replay it (default on)."""
+        _check(lib().nkr_g16_batch_set_graphs(self.h, 1 if enabled else 0))
+
+    def graph_active(self) -> bool:
+        v = ctypes.c_int32()
+        _check(lib().nkr_g16_batch_graph_active(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def close(self) -> None:
         if getattr(self, "h", None):

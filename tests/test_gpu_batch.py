@@ -64,3 +64,49 @@ def test_batch_rejects_bad_items(mix):
         N._check(N.lib().nkr_g16_batch_prove(b.h, None, 1))
     assert b.prove([]) == []
     b.close()
+
+
+def test_batch_graph_replays_match_plain(mix):
+    """A batch repeating the previous call's contexts is captured into a HIP
+    graph on its second call and replayed after; a batch of other contexts
+    runs eagerly and drops the graph.  Every call gets fresh witnesses (the
+    witnesses of one context rotate among its items) and fresh r, s, and
+    every proof of every call -- eager, captured and replayed, with eager
+    proofs and host synchronisations in between -- equals the plain path's."""
+    e, gold, cases = mix
+    b = N.Batch()
+    by_pv = {}
+    for i, (_, pv, _, _, w) in enumerate(cases):
+        by_pv.setdefault(id(pv), []).append(i)
+
+    def items_for(call, idx):
+        rs = circuits.random_scalars(2 * len(idx), 900 + call)
+        out = []
+        for n, i in enumerate(idx):
+            name, pv, k, c, _ = cases[i]
+            group = by_pv[id(pv)]
+            w = cases[group[(group.index(i) + call) % len(group)]][4]
+            out.append((name, pv, k, c, w, rs[2 * n], rs[2 * n + 1]))
+        return out
+
+    def check(call, idx):
+        its = items_for(call, idx)
+        outs = b.prove([(pv, w, r, s) for (_, pv, _, _, w, r, s) in its])
+        for (name, pv, k, c, w, r, s), (raw, proof) in zip(its, outs):
+            raw1, proof1 = pv.prove_blind(w, r, s)
+            assert np.array_equal(raw, raw1) and np.array_equal(proof, proof1), (call, name)
+            assert N.verify(k, w[1:1 + c.public_inputs], proof), (call, name)
+        return b.graph_active()
+
+    every = list(range(len(cases)))
+    sub = every[3:6]
+    assert not check(0, every)      # eager
+    assert check(1, every)          # captured, then replayed
+    assert check(2, every)          # replayed
+    assert check(3, every)          # replayed
+    assert not check(4, sub)        # another composition: eager, graph dropped
+    assert not check(5, every)      # eager again
+    assert check(6, every)          # captured again
+    b.set_graphs(False)
+    assert not check(7, every) and not check(8, every)  # capture off: eager
+    b.close()
