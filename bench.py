@@ -30,6 +30,7 @@ import argparse
 import ctypes
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -40,6 +41,7 @@ import numpy as np  # noqa: E402
 
 import circuits  # noqa: E402
 import autotune  # noqa: E402
+import monitor  # noqa: E402
 import nkr_g16 as N  # noqa: E402
 
 SEED = 0x4E4B52
@@ -300,6 +302,11 @@ def main() -> int:
                     help="only time the standalone operators (NTT, G1/G2 MSM) at the operator-configuration "
                          "sizes and print them as one JSON line")
     ap.add_argument("--op-reps", type=int, default=10)
+    ap.add_argument("--no-util", action="store_true",
+                    help="skip the utilisation qualification of the 2^20 proof (concurrency 1..16)")
+    ap.add_argument("--util-window", type=float, default=1.5,
+                    help="seconds of back-to-back calls per concurrency in the qualification")
+    ap.add_argument("--util-max-p", type=int, default=16)
     ap.add_argument("--batch-only", action="store_true",
                     help="only run the batch leg (the 64-proof mix, plain and batched) and print it as one JSON "
                          "line: the workload of the batch kernel trace")
@@ -477,6 +484,9 @@ def main() -> int:
         },
     }
     result["verified"] = bool(N.verify(pk, wit[1:1 + circ.public_inputs], proof))
+    if not args.no_util:
+        result["qualification"] = qualification_leg(prover, wit, rs, proof.copy(), args.util_window,
+                                                    args.util_max_p)
     h = None
     if not args.no_cpu_baseline:
         _, (_, _, _, h) = prover.prove(wit, intermediates=True)
@@ -582,6 +592,72 @@ def batch_leg(shapes, items, per_shape: int, reps: int) -> dict:
             "plain_proofs_per_s": round(n / plain_s, 2), "batched_proofs_per_s": round(n / batch_s, 2),
             "speedup": round(plain_s / batch_s, 3), "bit_exact_vs_plain": exact, "all_verified": verified,
             "reps": reps}
+
+
+def qualification_leg(prover, wit, rs, plain_proof, window_s: float, max_p: int) -> dict:
+    """The reference harness's utilisation gate (scripts/bench/qualify.py:
+    11-64): for concurrency p = 1, 2, 4, ... max_p (p proofs of the 2^20
+    context together -- p = 1 the plain call, p > 1 one batch call of p
+    proofs), run back to back for about window_s while a sampler reads the
+    device's utilisation every 100 ms; each attempt reports m / T_max (T_max
+    the slowest call), the trimmed utilisation summary and its status
+    (util_unqualified below 90 %).  The best attempt is the one with the most
+    proofs/s among those that reach the target."""
+    pci = N.device_pci_bus_id(0)
+    src_name, src = monitor.source_for(pci)
+    r, s_ = rs[0], rs[1]
+    attempts = []
+    p = 1
+    batch = N.Batch()
+    while True:
+        if p == 1:
+            def call():
+                return [prover.prove_blind(wit, r, s_)]
+        else:
+            args = [(prover, wit, r, s_)] * p
+
+            def call(args=args):
+                return batch.prove(args)
+        outs = call()  # warm-up: work buffers grow here
+        ok = all(np.array_equal(o[1], plain_proof) for o in outs)
+        sampler = monitor.Sampler(src) if src else None
+        if sampler:
+            sampler.start()
+        times = []
+        t_start = time.time()
+        while len(times) < 3 or time.time() - t_start < window_s:
+            t0 = time.perf_counter()
+            outs = call()
+            times.append(1000.0 * (time.perf_counter() - t0))
+        t_end = time.time()
+        samples = sampler.stop() if sampler else []
+        ok = ok and all(np.array_equal(o[1], plain_proof) for o in outs)
+        rec = monitor.qualify(times, samples, (t_start, t_end), m=p, ok=ok)
+        rec["concurrency"] = p
+        rec["calls"] = len(times)
+        if samples:
+            pw = [x.power_w for x in samples if x.power_w is not None]
+            ck = [x.clock_mhz for x in samples if x.clock_mhz is not None]
+            rec["power_w_mean"] = round(statistics.fmean(pw), 1) if pw else None
+            rec["gfx_clock_mhz_mean"] = round(statistics.fmean(ck), 1) if ck else None
+        attempts.append(rec)
+        log(f"qualification p={p}: {rec['throughput_proofs_s']} proofs/s (m/T_max), util mean "
+            f"{rec['util']['util_mean']} over {rec['util']['sample_count']} samples, {rec['status']}")
+        if p >= max_p:
+            break
+        p = min(2 * p, max_p)
+    batch.close()
+    qualified = [a for a in attempts if a["status"] == "ok"]
+    if qualified:
+        best = dict(max(qualified, key=lambda a: a["throughput_proofs_s"] or 0.0), qualified=True)
+    else:
+        best = dict(max(attempts, key=lambda a: (a["util"]["util_mean"] or -1.0, a["concurrency"])),
+                    qualified=False)
+    return {"source": src_name, "pci_bus_id": pci, "sample_ms": int(1000 * monitor.SAMPLE_S),
+            "target": monitor.TARGET, "trim_ratio": monitor.TRIM_RATIO,
+            "best": {k: best[k] for k in ("concurrency", "throughput_proofs_s", "t_max_ms", "status", "qualified")}
+            | {"util_mean": best["util"]["util_mean"]},
+            "attempts": attempts}
 
 
 def serving_leg(items, big, qps: float, n_req: int, verify_all: bool) -> dict:

@@ -377,6 +377,10 @@ int nkr_dev_free(void *ptr);
 int nkr_memcpy_htod(void *dst_dev, const void *src, uint64_t bytes);
 int nkr_memcpy_dtoh(void *dst, const void *src_dev, uint64_t bytes);
 int nkr_device_synchronize(void);
+/* PCI address of `device` ("dddd:bb:dd.f", NUL-terminated, at most len
+ * bytes): lets a harness match the device to what a management library or
+ * sysfs reports when it samples utilisation. */
+int nkr_device_pci_bus_id(int device, char *buf, int len);
 /* Plain streaming copy kernel (16-byte vectors): a known-traffic reference
  * for calibrating hardware byte counters.  bytes must be a multiple of 16. */
 int nkr_dev_copy(void *dst_dev, const void *src_dev, uint64_t bytes, void *stream);

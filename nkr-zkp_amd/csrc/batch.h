@@ -19,9 +19,10 @@
 // witnesses to the side stream as well, with the accumulations holding part
 // of the resident slots, measured no faster on MI355X: the batch is bound by
 // the accumulations' total work.)
-//   witnesses   host threads copy them into pinned memory in 1 MiB units
-//               while the calling thread issues each unit's host-to-device
-//               copy as soon as it is staged.
+//   witnesses   packed by the plain path's rule (values other than 0 and 1,
+//               bit masks for the rest) on host threads in 1 MiB chunks, each
+//               chunk's host-to-device copy issued as soon as it is packed,
+//               and expanded into every slot by one kernel.
 //   bit planes  three batch-wide regions (G1 over the witness, H, G2), one
 //               device-to-host copy each.
 //   graphs      a batch of the same contexts in the same order as the
@@ -96,7 +97,8 @@ private:
     MsmScheduleBatch sort_w_, sort_h_;
     MsmBatch<fq_t> batch_g1_;
     MsmBatch<fq2_t> batch_g2_;
-    PinnedStaging<uint8_t> witness_stage_, table_stage_;
+    WitnessBatchUpload wupload_;
+    PinnedStaging<uint8_t> table_stage_;
     DeviceBuffer<uint8_t> tables_;
     DeviceBuffer<Xyzz<fq_t>> planes_w_, planes_h_;  // bit planes: ar, bs1, zk of every proof; H of every proof
     DeviceBuffer<Xyzz<fq2_t>> planes_2_;            // bs2 of every proof

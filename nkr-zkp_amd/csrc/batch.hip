@@ -42,9 +42,6 @@ void parallel_for(size_t n, Fn fn)
 size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 constexpr uint32_t kBatchPiece = 32;
-// Witness staging unit (bytes): small enough that the first copies start
-// early, large enough that each copy moves at full PCIe rate.
-constexpr size_t kStageUnit = size_t(1) << 20;
 
 }  // namespace
 
@@ -134,6 +131,7 @@ void BatchProver::trim()
     sort_h_.release_device();
     batch_g1_.release_device();
     batch_g2_.release_device();
+    wupload_.release_device();
 }
 
 void BatchProver::prove(const std::vector<BatchItem> &items)
@@ -162,16 +160,13 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
     last_key_ = key;
     const bool graph = replay || capture;
 
-    // witnesses -> pinned staging -> slot buffers; bit planes of every proof
-    // in three batch-wide regions (G1 over the witness, H, G2) so that each
-    // reaches the host in one copy
-    size_t wbytes = 0;
+    // bit planes of every proof in three batch-wide regions (G1 over the
+    // witness, H, G2) so that each reaches the host in one copy
     uint32_t max_rows = 0, max_long = 0;
     std::vector<size_t> pw_off(K + 1, 0), ph_off(K + 1, 0), p2_off(K + 1, 0);
     for (size_t i = 0; i < K; ++i) {
         const Groth16Prover &g = *items[i].ctx;
         for (int m = 0; m < 3; ++m) max_long = std::max(max_long, g.views_[m].n_long);
-        wbytes += (size_t)g.cols_ * sizeof(fr_t);
         max_rows = std::max(max_rows, g.rows_);
         pw_off[i + 1] = pw_off[i] + 3 * g.wplan_.c;
         ph_off[i + 1] = ph_off[i] + g.hplan_.c;
@@ -188,47 +183,17 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         sl.pb.ensure(g.rows_);
         sl.pc.ensure(g.rows_);
     }
-    uint8_t *wst = witness_stage_.reuse(wbytes);
-    // Staging units of at most kStageUnit bytes, in witness order: host
-    // threads copy them into pinned memory while this thread issues each
-    // unit's host-to-device copy as soon as it is staged.  The upload is
+    // Witnesses: packed on host threads (values other than 0 and 1 travel,
+    // the rest as bit masks), each 1 MiB chunk's copy issued as soon as it is
+    // packed, then one expansion kernel into every slot.  The upload is
     // issued eagerly in every mode; a captured graph holds only the GPU work
-    // that follows it (stream order puts that work after the copies).
+    // that follows it (stream order puts that work after the expansion).
     {
-        struct Unit {
-            size_t item, at, bytes, stage_off;
-        };
-        std::vector<Unit> units;
-        size_t so = 0;
-        for (size_t i = 0; i < K; ++i) {
-            const size_t b = (size_t)items[i].ctx->cols_ * sizeof(fr_t);
-            for (size_t at = 0; at < b; at += kStageUnit) {
-                units.push_back({i, at, std::min(kStageUnit, b - at), so});
-                so += units.back().bytes;
-            }
-        }
-        const size_t U = units.size();
-        std::unique_ptr<std::atomic<uint8_t>[]> staged(new std::atomic<uint8_t>[U]);
-        for (size_t u = 0; u < U; ++u) staged[u].store(0, std::memory_order_relaxed);
-        std::thread stager([&] {
-            parallel_for(U, [&](size_t u) {
-                const Unit &un = units[u];
-                std::memcpy(wst + un.stage_off, static_cast<const uint8_t *>(items[un.item].witness) + un.at, un.bytes);
-                staged[u].store(1, std::memory_order_release);
-            });
-        });
-        struct Join {
-            std::thread &t;
-            ~Join() { t.join(); }
-        } join{stager};
-        for (size_t u = 0; u < U; ++u) {
-            while (!staged[u].load(std::memory_order_acquire)) std::this_thread::yield();
-            const Unit &un = units[u];
-            NKR_HIP(hipMemcpyAsync(reinterpret_cast<uint8_t *>(slots_[un.item]->z.get()) + un.at, wst + un.stage_off,
-                                   un.bytes, hipMemcpyHostToDevice, stream_));
-        }
+        std::vector<WitnessBatchUpload::Item> wi(K);
+        for (size_t i = 0; i < K; ++i)
+            wi[i] = {static_cast<const fr_t *>(items[i].witness), slots_[i]->z.get(), items[i].ctx->cols_};
+        wupload_.upload(wi, stream_);
     }
-    witness_stage_.uploaded(stream_);
 
     if (replay) {
         NKR_HIP(hipGraphLaunch(graph_exec_, stream_));

@@ -909,6 +909,14 @@ int nkr_device_synchronize(void)
     return guarded([&] { NKR_HIP(hipDeviceSynchronize()); });
 }
 
+int nkr_device_pci_bus_id(int device, char *buf, int len)
+{
+    return guarded([&] {
+        NKR_REQUIRE(buf && len >= 13, "buffer too small for a PCI address");
+        NKR_HIP(hipDeviceGetPCIBusId(buf, len, device));
+    });
+}
+
 int nkr_dev_copy(void *dst, const void *src, uint64_t bytes, void *stream)
 {
     return guarded([&] {

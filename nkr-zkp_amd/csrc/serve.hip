@@ -9,9 +9,13 @@ namespace nkr {
 uint64_t Server::request_bytes(const Groth16Prover &g)
 {
     const uint64_t polys = (uint64_t)g.cols() * sizeof(fr_t) + 3ull * g.rows() * sizeof(fr_t);
+    // the packed witness on the device (every value, at worst) and its masks,
+    // offsets and chunk entry
+    const uint64_t packed = (uint64_t)g.cols() * sizeof(fr_t) + ((uint64_t)g.cols() / 64 + 1) * 20 +
+                            ((uint64_t)g.cols() / WitnessUpload::kChunk + 1) * sizeof(WitnessBatchUpload::Chunk) + 16;
     const MsmPlan w = batch_plan(g.witness_plan()), h = batch_plan(g.h_plan());
     const uint64_t planes = (3ull * w.c + h.c) * sizeof(Xyzz<fq_t>) + (uint64_t)w.c * sizeof(Xyzz<fq2_t>);
-    return polys + planes + msm_schedule_bytes(w) + msm_schedule_bytes(h) + 3 * msm_job_bytes<fq_t>(w) +
+    return polys + packed + planes + msm_schedule_bytes(w) + msm_schedule_bytes(h) + 3 * msm_job_bytes<fq_t>(w) +
            msm_job_bytes<fq_t>(h) + msm_job_bytes<fq2_t>(w);
 }
 

@@ -22,6 +22,8 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
+#include <vector>
 
 #include "field.h"
 #include "runtime.h"
@@ -62,6 +64,46 @@ private:
     DeviceBuffer<uint8_t> meta_dev_;
     size_t meta_bytes_ = 0;
     hipEvent_t done_ = nullptr;  // the last upload's copies have been read
+    uint64_t last_bytes_ = 0;
+};
+
+// Several witnesses in one packed transfer (the batch prover's upload).  The
+// witnesses are laid end to end in one index space, each starting at a
+// multiple of 64 elements, and packed by the same rule as one witness; every
+// chunk of at most kChunk elements belongs to one witness.  One kernel
+// expands all chunks, each into its own witness's device array.  Buffers grow
+// to the largest batch and are reused.
+class WitnessBatchUpload {
+public:
+    struct Item {
+        const fr_t *host;
+        fr_t *z_dev;
+        uint32_t cols;
+    };
+    WitnessBatchUpload() = default;
+    WitnessBatchUpload(const WitnessBatchUpload &) = delete;
+    WitnessBatchUpload &operator=(const WitnessBatchUpload &) = delete;
+
+    // Enqueues on `stream` every item's transfer and its expansion; returns
+    // once every copy is issued.  The host arrays may be reused then.
+    void upload(const std::vector<Item> &items, hipStream_t stream);
+    uint64_t last_bytes() const { return last_bytes_; }
+    void release_device();
+
+    // Where each chunk's expansion writes (the last table of the metadata).
+    struct Chunk {
+        fr_t *z;         // the witness's device array, at the chunk's first element
+        uint32_t begin;  // first element in the shared index space (a multiple of 64)
+        uint32_t count;  // elements, at most kChunk
+    };
+
+private:
+    PinnedStaging<fr_t> vals_;
+    PinnedStaging<uint8_t> meta_;
+    DeviceBuffer<fr_t> pack_dev_;
+    DeviceBuffer<uint8_t> meta_dev_;
+    std::unique_ptr<std::atomic<uint32_t>[]> chunk_count_;
+    size_t chunk_cap_ = 0;
     uint64_t last_bytes_ = 0;
 };
 

@@ -16,9 +16,11 @@ constexpr uint32_t kWitnessGroup = 64;
 // `one`, masks[2g + 1] the values equal to `one`, and the marked values are
 // copied to vals[offsets[g]], vals[offsets[g] + 1], ..., where offsets[g] =
 // begin + the number of values the range packed before group g.  Returns the
-// number of values packed.
+// number of values packed.  host holds the elements from index host_first on
+// (element e is host[e - host_first]), so several witnesses laid end to end
+// share one index space.
 inline uint32_t witness_pack(const void *host, uint32_t begin, uint32_t end, const uint64_t one[4], void *vals,
-                             uint64_t *masks, uint32_t *offsets)
+                             uint64_t *masks, uint32_t *offsets, uint32_t host_first = 0)
 {
     const uint8_t *src = static_cast<const uint8_t *>(host);
     uint8_t *dst = static_cast<uint8_t *>(vals);
@@ -30,7 +32,7 @@ inline uint32_t witness_pack(const void *host, uint32_t begin, uint32_t end, con
         uint64_t other = 0, ones = 0;
         for (uint32_t e = g0; e < g1; ++e) {
             uint64_t w[4];
-            std::memcpy(w, src + (size_t)e * 32, sizeof(w));
+            std::memcpy(w, src + (size_t)(e - host_first) * 32, sizeof(w));
             if ((w[0] | w[1] | w[2] | w[3]) == 0) continue;
             const uint64_t bit = 1ull << (e - g0);
             if (((w[0] ^ one[0]) | (w[1] ^ one[1]) | (w[2] ^ one[2]) | (w[3] ^ one[3])) == 0) {

@@ -126,6 +126,7 @@ def lib() -> ctypes.CDLL:
             "nkr_memcpy_htod": [vp, vp, u64],
             "nkr_memcpy_dtoh": [vp, vp, u64],
             "nkr_device_synchronize": [],
+            "nkr_device_pci_bus_id": [i32, ctypes.c_char_p, i32],
             "nkr_g16_verify": [ctypes.POINTER(Vk), vp, u64, vp, ctypes.POINTER(ctypes.c_int)],
             "nkr_g16_setup": [ctypes.POINTER(Circuit), vp, i32, ctypes.POINTER(PkOut), ctypes.POINTER(VkOut)],
             "nkr_g16_batch_create": [i32, ctypes.POINTER(vp)],
@@ -177,6 +178,13 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(lib().nkr_device_count(ctypes.byref(n)))
     return n.value
+
+
+def device_pci_bus_id(device: int = 0) -> str:
+    """PCI address of `device`, lower case ("0000:75:00.0")."""
+    buf = ctypes.create_string_buffer(64)
+    _check(lib().nkr_device_pci_bus_id(device, buf, len(buf)))
+    return buf.value.decode().lower()
 
 
 FIELD_OPS = {"fr_mul": 0, "fr_add": 1, "fr_sub": 2, "fq_mul": 3, "fq_add": 4, "fq_sub": 5,

@@ -1,9 +1,14 @@
 // CPU check of the witness packing rule (nkr-zkp_amd/csrc/witness_pack.h).
 // argv: input file (n x 32 B elements), n, chunk (elements per packing
-// range), output file.  The ranges [c * chunk, min(n, (c + 1) * chunk)) are
-// packed one by one as the upload's workers pack them; the output holds the
-// packed values (n x 32 B, unused slots zero), then masks (2 x u64 per group),
-// offsets (u32 per group) and the packed count of each range (u32).
+// range), output file[, sizes].  The ranges [c * chunk, min(n, (c + 1) *
+// chunk)) are packed one by one as the upload's workers pack them; the output
+// holds the packed values (n x 32 B, unused slots zero), then masks (2 x u64
+// per group), offsets (u32 per group) and the packed count of each range
+// (u32).  With `sizes` (comma-separated witness lengths summing to the input's
+// elements), the input is several witnesses, each copied to a buffer of its
+// own and laid out as the batch upload lays them: from a multiple of 64 in one
+// index space, its ranges [base + k * chunk, ...) packed from its own buffer;
+// n is then the padded space and the output covers it.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -13,11 +18,23 @@
 
 int main(int argc, char **argv)
 {
-    if (argc != 5) return 2;
+    if (argc != 5 && argc != 6) return 2;
     const uint32_t n = (uint32_t)std::strtoul(argv[2], nullptr, 10);
     const uint32_t chunk = (uint32_t)std::strtoul(argv[3], nullptr, 10);
     if (chunk == 0 || chunk % nkr::kWitnessGroup) return 2;
-    std::vector<uint8_t> in((size_t)n * 32), vals((size_t)n * 32, 0);
+    std::vector<uint32_t> sizes;
+    if (argc == 6) {
+        for (const char *p = argv[5]; *p;) {
+            char *e = nullptr;
+            sizes.push_back((uint32_t)std::strtoul(p, &e, 10));
+            p = (*e == ',') ? e + 1 : e;
+        }
+    } else {
+        sizes.push_back(n);
+    }
+    size_t in_elems = 0;
+    for (uint32_t z : sizes) in_elems += z;
+    std::vector<uint8_t> in(in_elems * 32), vals((size_t)n * 32, 0);
     FILE *f = std::fopen(argv[1], "rb");
     if (!f || std::fread(in.data(), 1, in.size(), f) != in.size()) return 3;
     std::fclose(f);
@@ -27,9 +44,19 @@ int main(int argc, char **argv)
     // Fr one in Montgomery form: 2^256 mod r
     const uint64_t one[4] = {0xac96341c4ffffffbull, 0x36fc76959f60cd29ull, 0x666ea36f7879462eull,
                              0x0e0a77c19a07df2full};
-    for (uint32_t b = 0; b < n; b += chunk)
-        counts.push_back(nkr::witness_pack(in.data(), b, b + chunk < n ? b + chunk : n, one, vals.data(),
-                                           masks.data(), offsets.data()));
+    size_t src = 0;
+    uint32_t base = 0;
+    for (uint32_t z : sizes) {
+        std::vector<uint8_t> own(in.begin() + src * 32, in.begin() + (src + z) * 32);
+        for (uint32_t at = 0; at < z; at += chunk) {
+            const uint32_t b = base + at, e = base + (at + chunk < z ? at + chunk : z);
+            if (e > n) return 4;
+            counts.push_back(nkr::witness_pack(own.data(), b, e, one, vals.data(), masks.data(), offsets.data(), base));
+        }
+        src += z;
+        base = (base + z + 63) / 64 * 64;
+    }
+    if (argc == 6 && base != n) return 4;
     FILE *o = std::fopen(argv[4], "wb");
     if (!o) return 3;
     std::fwrite(vals.data(), 1, vals.size(), o);

@@ -88,3 +88,33 @@ def test_pack_round_trip(pack_bin, tmp_path, n, chunk, p_zero, p_one):
         assert counts[c] == other[sl].sum()
         assert np.array_equal(vals[c * chunk:c * chunk + counts[c]], w[sl][other[sl]])
     assert np.array_equal(expand(vals, masks, offsets, n), w)
+
+
+@pytest.mark.parametrize("sizes,chunk", [([1, 1], 64), ([63, 65, 64, 1], 64), ([1000, 5000, 3], 1024),
+                                         ([4096, 129, 4096], 4096), ([200, 7, 333], 128)])
+def test_pack_several_witnesses(pack_bin, tmp_path, sizes, chunk):
+    """The batch upload's layout: witnesses end to end from multiples of 64,
+    each packed from its own buffer; expanding the shared space gives every
+    witness back at its base, and the padding between them expands to zero."""
+    ws = [witness(z, 7 * z + i, 0.4, 0.3) for i, z in enumerate(sizes)]
+    bases, at = [], 0
+    for z in sizes:
+        bases.append(at)
+        at = (at + z + 63) // 64 * 64
+    n = at
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.ascontiguousarray(np.concatenate(ws), dtype=np.uint64).tofile(src)
+    subprocess.run([pack_bin, str(src), str(n), str(chunk), str(dst), ",".join(map(str, sizes))], check=True)
+    raw = np.fromfile(dst, dtype=np.uint8)
+    groups = n // 64
+    vals = raw[:n * 32].view(np.uint64).reshape(n, 4)
+    masks = raw[n * 32:n * 32 + 16 * groups].view(np.uint64).reshape(groups, 2)
+    offsets = raw[n * 32 + 16 * groups:n * 32 + 20 * groups].view(np.uint32)
+    counts = raw[n * 32 + 20 * groups:].view(np.uint32)
+    assert len(counts) == sum((z + chunk - 1) // chunk for z in sizes)
+    out = expand(vals, masks, offsets, n)
+    covered = np.zeros(n, dtype=bool)
+    for w, b, z in zip(ws, bases, sizes):
+        assert np.array_equal(out[b:b + z], w)
+        covered[b:b + z] = True
+    assert not out[~covered].any()
