@@ -321,28 +321,51 @@ NKR_HD Xyzz2_29<P> xyzz2_29_add(const Xyzz2_29<P> &a, const Xyzz2_29<P> &b)
     return r;
 }
 
-// acc + (x2, y2) over G2, complete.  x2, y2: the affine base from
-// fq2_29_from (below 32p).
+// The G2 accumulator as the mixed addition below reads and writes it: X(),
+// Y(), ZZ(), ZZZ() read a coordinate, set() replaces all four at once.  The
+// formula is written once over this interface, so a caller may keep some
+// coordinates outside the registers (the accumulation kernel holds Y and
+// ZZZ in LDS between their uses) with the same arithmetic.
 template <class P>
-NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &x2, const Fq2_29<P> &y2)
+struct Xyzz2_29Regs {
+    Xyzz2_29<P> p;
+    NKR_HD bool is_inf() const { return p.is_inf(); }
+    NKR_HD Fq2_29<P> X() const { return p.X; }
+    NKR_HD Fq2_29<P> Y() const { return p.Y; }
+    NKR_HD Fq2_29<P> ZZ() const { return p.ZZ; }
+    NKR_HD Fq2_29<P> ZZZ() const { return p.ZZZ; }
+    NKR_HD void set(const Fq2_29<P> &X, const Fq2_29<P> &Y, const Fq2_29<P> &ZZ, const Fq2_29<P> &ZZZ)
+    {
+        p.X = X;
+        p.Y = Y;
+        p.ZZ = ZZ;
+        p.ZZZ = ZZZ;
+    }
+    NKR_HD void set(const Xyzz2_29<P> &q) { p = q; }
+};
+
+// acc += (x2, y2) over G2, complete.  x2, y2: the affine base from
+// fq2_29_from (below 32p).  Each coordinate of acc is read where the
+// formula uses it (Y and ZZZ twice each) and all four are replaced at the end.
+template <class P, class Acc>
+NKR_HD void xyzz2_29_madd(Acc &acc, const Fq2_29<P> &x2, const Fq2_29<P> &y2)
 {
     if (acc.is_inf()) {
-        Xyzz2_29<P> r;
-        r.X = fq2_29_reduce_to_2p(x2);
-        r.Y = fq2_29_reduce_to_2p(y2);
-        r.ZZ.c0 = r.ZZZ.c0 = Fp29<P>::one();
-        r.ZZ.c1 = r.ZZZ.c1 = Fp29<P>::zero();
-        return r;
+        const Fq2_29<P> one = {Fp29<P>::one(), Fp29<P>::zero()};
+        acc.set(fq2_29_reduce_to_2p(x2), fq2_29_reduce_to_2p(y2), one, one);
+        return;
     }
-    const Fq2_29<P> U2 = fq2_29_mul<P, 2>(x2, acc.ZZ);
-    const Fq2_29<P> S2 = fq2_29_mul<P, 2>(y2, acc.ZZZ);
-    Fq2_29<P> Pd = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 8>(U2, acc.X));
-    Fq2_29<P> R = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(S2, acc.Y));
+    const Fq2_29<P> U2 = fq2_29_mul<P, 2>(x2, acc.ZZ());
+    const Fq2_29<P> S2 = fq2_29_mul<P, 2>(y2, acc.ZZZ());
+    Fq2_29<P> Pd = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 8>(U2, acc.X()));
+    Fq2_29<P> R = fq2_29_sub_if_ge<P, 4>(fq2_29_sub<P, 2>(S2, acc.Y()));
     if (fq2_29_is_zero_below_6p(Pd)) {
         // the base is +-(the accumulator): the canonical formulas decide
         if (fq2_29_is_zero_below_6p(R))
-            return xyzz2_29_from<P>(xyzz_dbl_affine(fq2_29_to(x2), fq2_29_to(y2)));
-        return Xyzz2_29<P>::infinity();
+            acc.set(xyzz2_29_from<P>(xyzz_dbl_affine(fq2_29_to(x2), fq2_29_to(y2))));
+        else
+            acc.set(Xyzz2_29<P>::infinity());
+        return;
     }
     fp29_opaque(Pd.c0);
     fp29_opaque(Pd.c1);
@@ -351,20 +374,30 @@ NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &
     const Fq2_29<P> PP = fq2_29_sqr<P, 6>(Pd);
     const Fq2_29<P> PPP = fq2_29_mul<P, 2>(Pd, PP);
     const Fq2_29<P> RR = fq2_29_sqr<P, 6>(R);
-    const Fq2_29<P> Q = fq2_29_mul<P, 2>(acc.X, PP);
-    Xyzz2_29<P> r;
-    r.ZZ = fq2_29_mul<P, 2>(acc.ZZ, PP);
+    const Fq2_29<P> Q = fq2_29_mul<P, 2>(acc.X(), PP);
+    const Fq2_29<P> ZZ3 = fq2_29_mul<P, 2>(acc.ZZ(), PP);
     const Fq2_29<P> Q2 = {fp29_add(Q.c0, Q.c0), fp29_add(Q.c1, Q.c1)};
-    r.X = fq2_29_sub<P, 4>(fq2_29_sub<P, 2>(RR, PPP), Q2);
-    const Fq2_29<P> T = fq2_29_sub<P, 8>(Q, r.X);
+    const Fq2_29<P> X3 = fq2_29_sub<P, 4>(fq2_29_sub<P, 2>(RR, PPP), Q2);
+    const Fq2_29<P> T = fq2_29_sub<P, 8>(Q, X3);
     const Fq2_29<P> N = fq2_29_sub<P, 2>({Fp29<P>::zero(), Fp29<P>::zero()}, PPP);
-    r.ZZZ = fq2_29_mul<P, 2>(acc.ZZZ, PPP);
+    const Fq2_29<P> ZZZ3 = fq2_29_mul<P, 2>(acc.ZZZ(), PPP);
     // Y3 = R T + Y N, per component one reduction of four partial products
     const Fp29<P> nT1 = fp29_sub<P, 10>(Fp29<P>::zero(), T.c1);
     const Fp29<P> nN1 = fp29_sub<P, 2>(Fp29<P>::zero(), N.c1);
-    r.Y.c0 = fp29_mul_add4(R.c0, T.c0, R.c1, nT1, acc.Y.c0, N.c0, acc.Y.c1, nN1);
-    r.Y.c1 = fp29_mul_add4(R.c0, T.c1, R.c1, T.c0, acc.Y.c0, N.c1, acc.Y.c1, N.c0);
-    return r;
+    const Fq2_29<P> Y1 = acc.Y();
+    Fq2_29<P> Y3;
+    Y3.c0 = fp29_mul_add4(R.c0, T.c0, R.c1, nT1, Y1.c0, N.c0, Y1.c1, nN1);
+    Y3.c1 = fp29_mul_add4(R.c0, T.c1, R.c1, T.c0, Y1.c0, N.c1, Y1.c1, N.c0);
+    acc.set(X3, Y3, ZZ3, ZZZ3);
+}
+
+// acc + (x2, y2) with the accumulator in registers.
+template <class P>
+NKR_HD Xyzz2_29<P> xyzz2_29_add_affine(const Xyzz2_29<P> &acc, const Fq2_29<P> &x2, const Fq2_29<P> &y2)
+{
+    Xyzz2_29Regs<P> a{acc};
+    xyzz2_29_madd<P>(a, x2, y2);
+    return a.p;
 }
 
 }  // namespace nkr

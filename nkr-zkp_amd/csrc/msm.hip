@@ -422,6 +422,58 @@ __global__ __launch_bounds__(1024) void k_piece_scan(const JobDev<F> *__restrict
     if (threadIdx.x == blockDim.x - 1) piece_base[J] = part[threadIdx.x];
 }
 
+// The G2 accumulator of one lane with X and ZZ in registers and Y and ZZZ in
+// LDS: each of Y and ZZZ is used twice per addition, far apart, and holding
+// them (36 registers) across the products would push the addition's working
+// set past the 256 registers a lane has at two waves per SIMD.  LDS holds
+// limb i of a coordinate's 18 at [i * kAccThreads + lane], so a wave's access
+// to one limb is 64 consecutive words (no bank conflicts).  The accesses are
+// volatile so that the compiler neither merges the two reads of a coordinate
+// nor keeps it in registers between them.
+// The pointers carry the LDS address space so that the accesses compile to
+// ds_read/ds_write rather than flat instructions.
+constexpr uint32_t kAccThreads = 256;
+typedef __attribute__((address_space(3))) volatile uint32_t lds_word_t;
+
+template <class P>
+struct Xyzz2_29Lds {
+    Fq2_29<P> x, zz;
+    lds_word_t *y, *zzz;  // this lane's column of each coordinate's 18 x kAccThreads words
+
+    __device__ static Fq2_29<P> load(const lds_word_t *s)
+    {
+        Fq2_29<P> r;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            r.c0.l[i] = s[i * kAccThreads];
+            r.c1.l[i] = s[(9 + i) * kAccThreads];
+        }
+        return r;
+    }
+    __device__ static void store(lds_word_t *s, const Fq2_29<P> &v)
+    {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            s[i * kAccThreads] = v.c0.l[i];
+            s[(9 + i) * kAccThreads] = v.c1.l[i];
+        }
+    }
+    __device__ bool is_inf() const { return zz.is_exact_zero(); }
+    __device__ Fq2_29<P> X() const { return x; }
+    __device__ Fq2_29<P> Y() const { return load(y); }
+    __device__ Fq2_29<P> ZZ() const { return zz; }
+    __device__ Fq2_29<P> ZZZ() const { return load(zzz); }
+    __device__ void set(const Fq2_29<P> &X, const Fq2_29<P> &Y, const Fq2_29<P> &ZZ, const Fq2_29<P> &ZZZ)
+    {
+        x = X;
+        zz = ZZ;
+        store(y, Y);
+        store(zzz, ZZZ);
+    }
+    __device__ void set(const Xyzz2_29<P> &q) { set(q.X, q.Y, q.ZZ, q.ZZZ); }
+    __device__ Xyzz2_29<P> get() const { return {x, load(y), zz, load(zzz)}; }
+};
+
 // Level 0 over all jobs: a thread sums one piece of signed bases of one
 // bucket into one partial, then moves on by the grid stride, so the launch
 // can be sized to the resident waves and every wave gets the same share.
@@ -445,18 +497,23 @@ __device__ __forceinline__ void accumulate_body(const JobDev<F> *__restrict__ jo
         partial_t<F> acc = partial_t<F>::infinity();
         if constexpr (std::is_same<F, fq2_t>::value) {
             // G2: the accumulator in the reduced-radix form as for G1
-            // (curve29.h, Fq2 bounds), stored in it; no prefetch.
-            Xyzz2_29<FqParams> acc29 = Xyzz2_29<FqParams>::infinity();
+            // (curve29.h, Fq2 bounds), stored in it; Y and ZZZ in LDS
+            // between their uses; no prefetch.
+            __shared__ uint32_t side[36 * kAccThreads];
+            Xyzz2_29Lds<FqParams> acc29{Fq2_29<FqParams>{}, Fq2_29<FqParams>{},
+                                        (lds_word_t *)(side + threadIdx.x),
+                                        (lds_word_t *)(side + 18 * kAccThreads + threadIdx.x)};
+            acc29.set(Xyzz2_29<FqParams>::infinity());
 #pragma unroll 1
             for (uint32_t e = beg; e < end; ++e) {
                 const uint32_t v = sorted[e];
                 const Affine<F> P = bases[v & kEntryMask];
                 if (!affine_is_inf(P)) {
                     const F y = (v >> 31) ? FieldOps<F>::neg(P.y) : P.y;
-                    acc29 = xyzz2_29_add_affine(acc29, fq2_29_from<FqParams>(P.x), fq2_29_from<FqParams>(y));
+                    xyzz2_29_madd<FqParams>(acc29, fq2_29_from<FqParams>(P.x), fq2_29_from<FqParams>(y));
                 }
             }
-            acc = acc29;
+            acc = acc29.get();
         } else if (beg < end) {
             // G1: the accumulator stays in the reduced-radix form
             // (curve29.h: one v_mad_u64_u32 per product term, no carries)
@@ -491,9 +548,10 @@ __global__ __launch_bounds__(256) void k_accumulate(const JobDev<F> *__restrict_
 }
 
 // The G2 addition's working set exceeds 256 registers, which would leave one
-// wave per SIMD; held to two waves, its spills to scratch cost less than the
-// second wave gains (G2 accumulation of the dense 2^20 Poseidon-style proof
-// 10.3 -> 8.5 ms on MI355X).
+// wave per SIMD; held to two waves, its spills cost less than the second wave
+// gains (G2 accumulation of the dense 2^20 Poseidon-style proof 10.3 -> 8.5
+// ms on MI355X), and with the accumulator's Y and ZZZ in LDS (36 KiB per
+// workgroup) the spills fall from 280 to 80 bytes per lane.
 template <>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_accumulate<fq2_t>(
     const JobDev<fq2_t> *__restrict__ jobs, const uint32_t *__restrict__ piece_base, uint32_t J)
@@ -1077,7 +1135,7 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
     const int acc_stage = std::is_same<F, fq2_t>::value ? kStageAccumulateG2 : kStageAccumulate;
     if (timer) timer->begin(acc_stage, stream);
     const uint64_t blocks = std::min<uint64_t>(div_up(thread_total_, 256), resident_blocks<k_accumulate<F>>());
-    k_accumulate<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, stream>>>(jd, thread_base_dev_.get(), J);
+    k_accumulate<F><<<(uint32_t)std::max<uint64_t>(blocks, 1), kAccThreads, 0, stream>>>(jd, thread_base_dev_.get(), J);
     NKR_LAUNCH_CHECK();
     if (timer) {
         timer->end(acc_stage, stream);
