@@ -110,3 +110,41 @@ def test_batch_graph_replays_match_plain(mix):
     b.set_graphs(False)
     assert not check(7, every) and not check(8, every)  # capture off: eager
     b.close()
+
+
+def test_batch_upload_chunk_edges():
+    """The batch's packed upload lays witnesses end to end from multiples of
+    64 and cuts them into chunks of 2^15 elements: witnesses of 35841 columns
+    (two chunks, the second partial, a length that is not a multiple of 64)
+    between small ones, one of them dense, one with a chunk of zeros and one
+    with a run of ones, each give the proof the single-witness upload gives
+    (the valid one also verifies)."""
+    import pyoracle as O
+
+    base = circuits.matmul(32, 11)
+    live = 1 + 3 * 32 * 32 + 32 * 32 * 32
+    assert live % 64 and live > 1 << 15 and not base.witness[live:].any()
+    c = circuits.R1cs(base.rows, live, base.public_inputs, base.a, base.b, base.c,
+                      np.ascontiguousarray(base.witness[:live]))
+    small = circuits.fibonacci(300, 1)
+    k, ks = keyed(c, 301), keyed(small, 302)
+    pv, pvs = N.Prover(c, k), N.Prover(small, ks)
+    one = np.array(O.int_to_words(O.to_mont_int(1, O.R)), dtype=np.uint64)
+    w_zero = c.witness.copy()
+    w_zero[1 << 14:(1 << 14) + (1 << 15)] = 0
+    w_ones = c.witness.copy()
+    w_ones[100:30000] = one
+    ws = [(pv, c.witness), (pvs, small.witness), (pv, w_zero), (pv, w_ones), (pvs, small.witness)]
+    rs = circuits.random_scalars(2 * len(ws), 303)
+    items = [(p, w, rs[2 * i], rs[2 * i + 1]) for i, (p, w) in enumerate(ws)]
+    b = N.Batch()
+    for rep in range(2):  # the second call reuses the grown buffers
+        outs = b.prove(items)
+        for i, (p, w, r, s) in enumerate(items):
+            raw, proof = p.prove_blind(w, r, s)
+            assert np.array_equal(outs[i][0], raw), (rep, i)
+            assert np.array_equal(outs[i][1], proof), (rep, i)
+    assert N.verify(k, c.witness[1:1 + c.public_inputs], outs[0][1])
+    b.close()
+    pv.close()
+    pvs.close()
