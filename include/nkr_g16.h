@@ -193,6 +193,15 @@ int nkr_g16_server_create(int device, uint32_t workers, uint32_t max_batch, uint
 int nkr_g16_server_submit(nkr_g16_server *server, nkr_g16_ctx *ctx, const void *witness, const void *r,
                           const void *s, void *raw_out, void *proof_out, uint64_t *ticket);
 int nkr_g16_server_wait(nkr_g16_server *server, uint64_t ticket, double *latency_ms);
+/* Warm-up before traffic (no counterpart in the reference): every worker
+ * proves the `count` items once as one batch (outputs written to the items'
+ * buffers), so that its work buffers are already sized for such requests
+ * instead of growing on the first one, which would then pay the device
+ * allocations.  Waits until no batch is in flight; requests submitted
+ * meanwhile wait for it.  count <= max_batch and the items' request bytes
+ * must fit one worker's share.  Not concurrently with another warm-up or
+ * with nkr_g16_server_destroy. */
+int nkr_g16_server_warm(nkr_g16_server *server, const nkr_g16_batch_item *items, uint64_t count);
 int nkr_g16_server_request_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes);
 int nkr_g16_server_stats(const nkr_g16_server *server, uint64_t *stats);
 int nkr_g16_server_memory(const nkr_g16_server *server, uint64_t *memory);

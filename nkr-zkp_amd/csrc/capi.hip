@@ -347,6 +347,24 @@ int nkr_g16_server_wait(nkr_g16_server *server, uint64_t ticket, double *latency
     });
 }
 
+int nkr_g16_server_warm(nkr_g16_server *server, const nkr_g16_batch_item *items, uint64_t count)
+{
+    return guarded([&] {
+        NKR_REQUIRE(server && items, "null argument");
+        NKR_REQUIRE(count >= 1 && count <= 16384, "a warm-up holds 1 .. 16384 items");
+        std::vector<BatchItem> v(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            const nkr_g16_batch_item &it = items[i];
+            NKR_REQUIRE(it.ctx && it.r && it.s, "null warm-up item member");
+            fr_t r, s;
+            std::memcpy(&r, it.r, sizeof(r));
+            std::memcpy(&s, it.s, sizeof(s));
+            v[i] = BatchItem{it.ctx->prover.get(), it.witness, r, s, it.raw_out, it.proof_out};
+        }
+        server->server->warm(v);
+    });
+}
+
 int nkr_g16_server_request_bytes(const nkr_g16_ctx *ctx, uint64_t *bytes)
 {
     return guarded([&] {

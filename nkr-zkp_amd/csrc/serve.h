@@ -57,6 +57,16 @@ public:
     // (submission to completion) or throws the error its batch raised.
     double wait(uint64_t ticket);
 
+    // Warm-up before traffic: every worker proves `items` once, as one batch
+    // (the outputs are written to the items' buffers), so that each worker's
+    // buffers already have the size those requests need; otherwise a worker
+    // grows them on its first such request, which then pays the device
+    // allocations.  Waits until no batch is in flight; requests submitted
+    // meanwhile wait for the warm-up.  The items' charges must fit one
+    // worker's share and max_batch; calls must not overlap each other or the
+    // server's destruction.
+    void warm(const std::vector<BatchItem> &items);
+
     // Device bytes a request on this context is charged at admission.
     static uint64_t request_bytes(const Groth16Prover &ctx);
 
@@ -101,6 +111,7 @@ private:
     uint64_t next_ticket_ = 1;
     uint64_t in_use_ = 0;
     bool stop_ = false;
+    bool warming_ = false;  // workers admit nothing while warm() runs
     Stats stats_;
     std::vector<std::unique_ptr<BatchProver>> provers_;
     std::vector<std::thread> threads_;

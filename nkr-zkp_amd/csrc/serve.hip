@@ -84,6 +84,58 @@ double Server::wait(uint64_t ticket)
     return d.latency_ms;
 }
 
+void Server::warm(const std::vector<BatchItem> &items)
+{
+    NKR_REQUIRE(!items.empty(), "nothing to warm with");
+    NKR_REQUIRE(items.size() <= max_batch_, "more warm-up items than max_batch");
+    uint64_t bytes = 0;
+    for (const BatchItem &it : items) {
+        NKR_REQUIRE(it.ctx && it.witness && it.raw_out && it.proof_out, "null warm-up item member");
+        NKR_REQUIRE(it.ctx->device() == device_, "context prepared on another device");
+        NKR_REQUIRE(it.ctx->can_blind(), "the context was prepared without alpha/beta/delta, which blinding needs");
+        bytes += request_bytes(*it.ctx);
+    }
+    NKR_REQUIRE(bytes <= share_, "warm-up items need " + std::to_string(bytes) +
+                                     " bytes of work memory, more than a worker's share of " +
+                                     std::to_string(share_));
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        NKR_REQUIRE(!stop_, "server is shutting down");
+        NKR_REQUIRE(!warming_, "a warm-up is already running");
+        warming_ = true;
+        cv_done_.wait(lk, [&] { return in_use_ == 0; });
+    }
+    struct Resume {
+        Server *s;
+        ~Resume()
+        {
+            {
+                std::lock_guard<std::mutex> lk(s->mu_);
+                s->warming_ = false;
+            }
+            s->cv_work_.notify_all();
+        }
+    } resume{this};
+    // the workers are idle and admit nothing: their batch provers are free
+    for (size_t i = 0; i < provers_.size(); ++i) {
+        BatchProver &p = *provers_[i];
+        if (p.device_bytes() + bytes > share_) p.trim();
+        p.prove(items);
+        uint64_t held = p.device_bytes();
+        const uint64_t peak = held;
+        if (held > share_) {
+            p.trim();
+            held = p.device_bytes();
+        }
+        std::lock_guard<std::mutex> lk(mu_);
+        held_[i] = peak;
+        uint64_t sum = 0;
+        for (uint64_t h : held_) sum += h;
+        peak_held_ = std::max(peak_held_, sum);
+        held_[i] = held;
+    }
+}
+
 Server::Stats Server::stats() const
 {
     std::lock_guard<std::mutex> lk(mu_);
@@ -149,7 +201,7 @@ void Server::worker_loop(uint32_t index)
             std::unique_lock<std::mutex> lk(mu_);
             // admit pending requests in arrival order while their charges fit
             // this worker's share (every request fits a share: submit checks)
-            cv_work_.wait(lk, [&] { return !pending_.empty() || stop_; });
+            cv_work_.wait(lk, [&] { return (!pending_.empty() && !warming_) || stop_; });
             if (stop_ && pending_.empty()) return;
             while (!pending_.empty() && batch.size() < max_batch_ && bytes + pending_.front().bytes <= share_) {
                 bytes += pending_.front().bytes;

@@ -133,6 +133,7 @@ def lib() -> ctypes.CDLL:
             "nkr_g16_batch_prove": [vp, ctypes.POINTER(BatchItem), u64],
             "nkr_g16_batch_destroy": [vp],
             "nkr_g16_batch_set_graphs": [vp, i32],
+            "nkr_g16_server_warm": [vp, ctypes.POINTER(BatchItem), u64],
             "nkr_g16_batch_graph_active": [vp, ctypes.POINTER(i32)],
             "nkr_g16_server_create": [i32, u32, u32, u64, ctypes.POINTER(vp)],
             "nkr_g16_server_submit": [vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(u64)],
@@ -449,6 +450,25 @@ class Server:
         finally:
             w, r, s, raw, proof = self._live.pop(ticket)
         return raw, proof, ms.value
+
+    def warm(self, items):
+        """Every worker proves items ((Prover, witness, r, s) tuples) once, as
+        one batch, so its buffers are sized before traffic; returns the
+        outputs as Batch.prove does."""
+        keep, outs = [], []
+        arr = (BatchItem * max(len(items), 1))()
+        for i, (pv, w, r, s) in enumerate(items):
+            w = _words(w, 4)
+            assert w.shape[0] == pv.cols
+            r = np.ascontiguousarray(r, dtype=np.uint64).reshape(4)
+            s = np.ascontiguousarray(s, dtype=np.uint64).reshape(4)
+            raw = np.zeros(72, dtype=np.uint64)
+            proof = np.zeros(48, dtype=np.uint64)
+            keep += [w, r, s]
+            outs.append((raw, proof))
+            arr[i] = BatchItem(pv.h.value, _p(w), _p(r), _p(s), _p(raw), _p(proof))
+        _check(lib().nkr_g16_server_warm(self.h, arr, len(items)))
+        return outs
 
     def stats(self) -> dict:
         st = (ctypes.c_uint64 * 4)()

@@ -173,3 +173,49 @@ def test_request_without_blinding_keys_fails_alone(contexts):
     for g, want in zip(got, alone):
         assert np.array_equal(g, want)
     assert st["proofs"] == len(reqs) + 1
+
+
+def test_warm_up_sizes_every_worker(contexts):
+    """nkr_g16_server_warm: every worker proves the warm-up items once (the
+    outputs equal the proofs made alone), each worker then holds buffers for
+    them (within its share), requests submitted during the warm-up wait for
+    it and still equal their proofs made alone, and invalid warm-ups are
+    rejected without stopping the server."""
+    reqs = requests(contexts, 12, 57)
+    alone = [pv.prove_blind(w, r, s) for (_, _, _, pv, w, r, s) in reqs]
+    srv = N.Server(workers=3, max_batch=4)
+    try:
+        assert srv.memory()["held"] == 0
+        items = [(q[3], q[4], q[5], q[6]) for q in reqs[:3]]
+        outs = srv.warm(items)
+        for (raw, proof), (raw1, proof1) in zip(outs, alone[:3]):
+            assert np.array_equal(raw, raw1) and np.array_equal(proof, proof1)
+        mem = srv.memory()
+        # each of the 3 workers holds at least the items' witness buffers
+        witness_bytes = sum(q[3].cols * 32 for q in reqs[:3])
+        assert mem["held"] >= 3 * witness_bytes, mem
+        assert mem["held"] <= mem["budget"] and mem["peak_held"] <= mem["budget"], mem
+        # a second warm-up while requests arrive: they wait, then complete
+        tickets = []
+        done = threading.Event()
+
+        def submit_during():
+            for q in reqs[3:]:
+                tickets.append(srv.submit(q[3], q[4], q[5], q[6]))
+            done.set()
+
+        th = threading.Thread(target=submit_during)
+        th.start()
+        srv.warm(items[:1])
+        th.join()
+        assert done.is_set()
+        got = [srv.wait(t) for t in tickets]
+        for (raw, proof, _), (raw1, proof1) in zip(got, alone[3:]):
+            assert np.array_equal(raw, raw1) and np.array_equal(proof, proof1)
+        with pytest.raises(N.NkrError):
+            srv.warm(items * 2)  # more than max_batch
+        # the server still serves after a rejected warm-up
+        t = srv.submit(reqs[0][3], reqs[0][4], reqs[0][5], reqs[0][6])
+        assert np.array_equal(srv.wait(t)[1], alone[0][1])
+    finally:
+        srv.close()
