@@ -350,7 +350,7 @@ void BatchProver::prove(const std::vector<BatchItem> &items)
         const BatchItem &it = items[i];
         const Groth16Prover &g = *it.ctx;
         const Groth16Prover::BlindJob job{it.r, it.s, it.proof_out};
-        g.finish(hw + pw_off[i], hh + ph_off[i], h2 + p2_off[i], &job, &pre[i], it.raw_out);
+        g.finish_serial(hw + pw_off[i], hh + ph_off[i], h2 + p2_off[i], &job, &pre[i], it.raw_out);
     });
 }
 

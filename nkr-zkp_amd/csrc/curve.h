@@ -218,6 +218,37 @@ NKR_HD Jacobian<F> xyzz_to_jacobian_affine(const Xyzz<F> &p)
     return r;
 }
 
+// xyzz_to_jacobian_affine of n <= 8 points with one inversion shared by all
+// (Montgomery's simultaneous inversion of the ZZ * ZZZ products; infinity
+// takes no part): the same points, so the same canonical coordinates.
+template <class F>
+NKR_HD void xyzz_to_jacobian_affine_batch(const Xyzz<F> *in, Jacobian<F> *out, int n)
+{
+    using O = FieldOps<F>;
+    F d[8], prefix[8];
+    F acc = O::one();
+    for (int i = 0; i < n; ++i) {
+        d[i] = in[i].is_inf() ? O::one() : O::mul(in[i].ZZ, in[i].ZZZ);
+        prefix[i] = acc;
+        acc = O::mul(acc, d[i]);
+    }
+    F inv = O::inv(acc);  // 1 / (d_0 ... d_{n-1})
+    for (int i = n - 1; i >= 0; --i) {
+        const F t = O::mul(inv, prefix[i]);  // 1 / d_i
+        inv = O::mul(inv, d[i]);
+        Jacobian<F> &r = out[i];
+        if (in[i].is_inf()) {
+            r.X = O::zero();
+            r.Y = O::one();
+            r.Z = O::zero();
+            continue;
+        }
+        r.X = O::mul(in[i].X, O::mul(t, in[i].ZZZ));
+        r.Y = O::mul(in[i].Y, O::mul(t, in[i].ZZ));
+        r.Z = O::one();
+    }
+}
+
 template <class F>
 NKR_HD Affine<F> xyzz_to_affine(const Xyzz<F> &p)
 {

@@ -269,6 +269,9 @@ private:
 // Host: sum_j 2^j U_j, then the reference's affine form (Z = 1, or (0, 1, 0)).
 template <class F>
 Jacobian<F> msm_finish(const Xyzz<F> *planes_host, uint32_t c);
+// The same sum left in XYZZ form (not normalised).
+template <class F>
+Xyzz<F> msm_sum(const Xyzz<F> *planes_host, uint32_t c);
 
 // Whole-vector convenience used by the standalone operator API.
 template <class F>

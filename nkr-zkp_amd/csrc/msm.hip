@@ -1202,14 +1202,20 @@ void MsmBatch<F>::run(const std::vector<MsmJob<F>> &jobs, hipStream_t stream, St
 }
 
 template <class F>
-Jacobian<F> msm_finish(const Xyzz<F> *planes, uint32_t c)
+Xyzz<F> msm_sum(const Xyzz<F> *planes, uint32_t c)
 {
     Xyzz<F> acc = planes[c - 1];
     for (int j = (int)c - 2; j >= 0; --j) {
         acc = xyzz_dbl(acc);
         acc = xyzz_add(acc, planes[j]);
     }
-    return xyzz_to_jacobian_affine(acc);
+    return acc;
+}
+
+template <class F>
+Jacobian<F> msm_finish(const Xyzz<F> *planes, uint32_t c)
+{
+    return xyzz_to_jacobian_affine(msm_sum(planes, c));
 }
 
 template <class F>
@@ -1240,6 +1246,8 @@ template class MsmContext<fq_t>;
 template class MsmContext<fq2_t>;
 template Jacobian<fq_t> msm_finish(const Xyzz<fq_t> *, uint32_t);
 template Jacobian<fq2_t> msm_finish(const Xyzz<fq2_t> *, uint32_t);
+template Xyzz<fq_t> msm_sum(const Xyzz<fq_t> *, uint32_t);
+template Xyzz<fq2_t> msm_sum(const Xyzz<fq2_t> *, uint32_t);
 template uint64_t msm_job_bytes<fq_t>(const MsmPlan &);
 template uint64_t msm_job_bytes<fq2_t>(const MsmPlan &);
 

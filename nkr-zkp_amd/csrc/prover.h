@@ -105,6 +105,14 @@ private:
     // the calling thread, as the batch path's per-proof workers do).
     void finish_g1(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const BlindJob *blind, const BlindPre *pre,
                    void *raw_out, HostHelpers *helpers) const;
+    // finish on the calling thread with less arithmetic, for callers that
+    // finish many proofs over their own threads: the same outputs (the same
+    // points, whose canonical affine coordinates do not depend on how they
+    // were reached), with every point kept in XYZZ form until one inversion
+    // per group (ar, bs1, zk, qz, A, C; bs2, B) and s*ar + r*bs1 as one
+    // multiplication sharing its doublings (g1_mul2_glv).
+    void finish_serial(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const Xyzz<fq2_t> *p2, const BlindJob *blind,
+                       const BlindPre *pre, void *raw_out) const;
     friend class BatchProver;
 
     // Fixed-base comb tables of alpha_1, beta_1, delta_1 (G1) and delta_2 (G2)

@@ -290,6 +290,43 @@ void Groth16Prover::finish(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const X
     finish_g1(p1w, p1h, blind, pre, raw_out, nullptr);
 }
 
+void Groth16Prover::finish_serial(const Xyzz<fq_t> *p1w, const Xyzz<fq_t> *p1h, const Xyzz<fq2_t> *p2,
+                                  const BlindJob *blind, const BlindPre *pre, void *raw_out) const
+{
+    const uint32_t cw = wplan_.c;
+    // raw proof: ar, bs1, zk, qz (G1) and bs2 (G2); blinded: A, C (G1), B (G2)
+    g1_xyzz g1[6];
+    g2_xyzz g2[2];
+    g1[0] = msm_sum(p1w + 0 * cw, cw);
+    g1[1] = msm_sum(p1w + 1 * cw, cw);
+    g1[2] = msm_sum(p1w + 2 * cw, cw);
+    g1[3] = msm_sum(p1h, hplan_.c);
+    g2[0] = msm_sum(p2, cw);
+    int n1 = 4, n2 = 1;
+    if (blind) {
+        g1[4] = xyzz_add(pre->A0, g1[0]);
+        g2[1] = xyzz_add(pre->B0, g2[0]);
+        g1_xyzz C = xyzz_add(g1[3], g1[2]);
+        C = xyzz_add(C, pre->C0);
+        C = xyzz_add(C, g1_mul2_glv(g1[0], blind->s, g1[1], blind->r));
+        g1[5] = C;
+        n1 = 6;
+        n2 = 2;
+    }
+    g1_jac a1[6];
+    g2_jac a2[2];
+    xyzz_to_jacobian_affine_batch(g1, a1, n1);
+    xyzz_to_jacobian_affine_batch(g2, a2, n2);
+    uint8_t *out = static_cast<uint8_t *>(raw_out);
+    for (int i = 0; i < 4; ++i) std::memcpy(out + 96 * i, &a1[i], 96);
+    std::memcpy(out + 384, &a2[0], 192);
+    if (!blind) return;
+    uint8_t *proof = static_cast<uint8_t *>(blind->proof_out);
+    std::memcpy(proof, &a1[4], 96);
+    std::memcpy(proof + 96, &a2[1], 192);
+    std::memcpy(proof + 288, &a1[5], 96);
+}
+
 void Groth16Prover::finish_g2(const Xyzz<fq2_t> *p2, const BlindJob *blind, const BlindPre *pre,
                               void *raw_out) const
 {

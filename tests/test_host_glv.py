@@ -53,3 +53,41 @@ def test_glv_matches_oracle(glv_bin):
                 assert not want[i][8:12].any(), i
             else:
                 assert np.array_equal(got[i], want[i]), i
+
+
+def test_joint_glv_and_batched_normalisation(glv_bin):
+    """k P + l Q as one multiplication of four GLV half-scalars
+    (g1_mul2_glv), and k P, l Q on their own, normalised together with one
+    shared inversion (xyzz_to_jacobian_affine_batch): all three equal the
+    oracle's, including sums and terms at infinity."""
+    g = O.g1_generator()
+    p = O.g1_mul(g, O.fr_array([123456789]))[0]
+    p = O.g1_add(O.g1_add(p, g), O.g1_neg(g))  # Z != 1
+    q = O.g1_mul(g, O.fr_array([987654321]))[0]
+    rng = np.random.default_rng(0x2B1)
+    cases = []
+    for _ in range(40):
+        k, l = (int(rng.integers(1, 1 << 62)) * int(rng.integers(1, 1 << 62)) * 7919) % O.R, \
+               (int(rng.integers(1, 1 << 62)) * int(rng.integers(1, 1 << 62)) * 104729) % O.R
+        cases.append((p, k, q, l))
+    k = 0xDEADBEEF12345
+    cases += [(p, 0, q, 5), (p, 7, q, 0), (p, 0, q, 0), (p, k, p, O.R - k), (q, LAMBDA, q, 1),
+              (p, O.R - 1, q, O.R - 1), (p, 1 << 128, q, (1 << 127) + 3)]
+    lines = []
+    for (a, ka, b, kb) in cases:
+        kw, lw = O.fr_array([ka]), O.fr_array([kb])
+        lines.append(" ".join(str(int(v)) for v in list(a) + list(kw[0]) + list(b) + list(lw[0])))
+    res = subprocess.run([glv_bin, "pair"], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                         check=True)
+    got = np.array([[int(v) for v in l.split()] for l in res.stdout.strip().splitlines()], dtype=np.uint64)
+    assert got.shape == (len(cases), 36)
+    for i, (a, ka, b, kb) in enumerate(cases):
+        pa = O.g1_mul(a, O.fr_array([ka]))[0]
+        pb = O.g1_mul(b, O.fr_array([kb]))[0]
+        for j, want in enumerate((O.g1_add(pa, pb), pa, pb)):
+            w = O.g1_to_affine(np.array([want]))[0]
+            g_ = got[i, 12 * j:12 * j + 12]
+            if not g_.any():
+                assert not w[8:12].any(), (i, j)
+            else:
+                assert np.array_equal(g_, w), (i, j)
