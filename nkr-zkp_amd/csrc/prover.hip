@@ -342,16 +342,18 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const Xyzz<fq_t> *p1h, const
 {
     const uint32_t cw = wplan_.c;
     uint8_t *out = static_cast<uint8_t *>(raw_out);
-    // Three independent parts: zk and H; bs1 and r*bs1; ar, A and s*ar.
-    g1_jac zk, qz, ar, bs1;
+    // Three independent parts: zk and H; bs1 and r*bs1; ar, A and s*ar.  The
+    // points stay in XYZZ form until one inversion shared by all six (ar,
+    // bs1, zk, qz, A, C).
+    g1_xyzz pts[6];
     g1_xyzz s_ar = g1_xyzz::infinity(), r_bs1 = g1_xyzz::infinity();
     auto part_zq = [&] {
-        zk = msm_finish(p1 + 2 * cw, cw);
-        qz = msm_finish(p1h, hplan_.c);
+        pts[2] = msm_sum(p1 + 2 * cw, cw);
+        pts[3] = msm_sum(p1h, hplan_.c);
     };
     auto part_b = [&] {
-        bs1 = msm_finish(p1 + 1 * cw, cw);
-        if (blind) r_bs1 = g1_mul_glv(jacobian_to_xyzz(bs1), blind->r);
+        pts[1] = msm_sum(p1 + 1 * cw, cw);
+        if (blind) r_bs1 = g1_mul_glv(pts[1], blind->r);
     };
     // the helpers' tasks refer to this frame: if anything below throws, the
     // guard waits for them before the frame goes away
@@ -371,12 +373,10 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const Xyzz<fq_t> *p1h, const
         helpers->run(1, part_b);
         join.h = helpers;
     }
-    ar = msm_finish(p1 + 0 * cw, cw);
+    pts[0] = msm_sum(p1 + 0 * cw, cw);
     if (blind) {
-        const g1_xyzz xar = jacobian_to_xyzz(ar);
-        const g1_jac A = xyzz_to_jacobian_affine(xyzz_add(pre->A0, xar));
-        std::memcpy(static_cast<uint8_t *>(blind->proof_out), &A, 96);
-        s_ar = g1_mul_glv(xar, blind->s);
+        pts[4] = xyzz_add(pre->A0, pts[0]);
+        s_ar = g1_mul_glv(pts[0], blind->s);
     }
     if (helpers) {
         join.h = nullptr;
@@ -391,16 +391,20 @@ void Groth16Prover::finish_g1(const Xyzz<fq_t> *p1, const Xyzz<fq_t> *p1h, const
         part_b();
         part_zq();
     }
-    std::memcpy(out + 0, &ar, 96);
-    std::memcpy(out + 96, &bs1, 96);
-    std::memcpy(out + 192, &zk, 96);
-    std::memcpy(out + 288, &qz, 96);
+    int n = 4;
+    if (blind) {
+        g1_xyzz C = xyzz_add(pts[3], pts[2]);
+        C = xyzz_add(C, pre->C0);
+        C = xyzz_add(C, xyzz_add(s_ar, r_bs1));
+        pts[5] = C;
+        n = 6;
+    }
+    g1_jac aff[6];
+    xyzz_to_jacobian_affine_batch(pts, aff, n);
+    for (int i = 0; i < 4; ++i) std::memcpy(out + 96 * i, &aff[i], 96);
     if (!blind) return;
-    g1_xyzz C = xyzz_add(jacobian_to_xyzz(qz), jacobian_to_xyzz(zk));
-    C = xyzz_add(C, pre->C0);
-    C = xyzz_add(C, xyzz_add(s_ar, r_bs1));
-    const g1_jac Cj = xyzz_to_jacobian_affine(C);
-    std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 288, &Cj, 96);
+    std::memcpy(static_cast<uint8_t *>(blind->proof_out), &aff[4], 96);
+    std::memcpy(static_cast<uint8_t *>(blind->proof_out) + 288, &aff[5], 96);
 }
 
 void Groth16Prover::run(const fr_t *z, void *raw_out, void *az, void *bz, void *cz, void *h, const BlindJob *blind)
